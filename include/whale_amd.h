@@ -1,0 +1,171 @@
+/*
+ * whale_amd.h -- C ABI of the MI355X (gfx950) amalgamated-likelihood DP.
+ *
+ * This is the drop-in boundary for Whale.jl's likelihood path.  The reference
+ * has no FFI of its own; its boundary is Julia dispatch on
+ *     Distributions.logpdf(::WhaleModel, ::CCD)            src/core.jl:39-43
+ *     logpdf!(::WhaleModel, ::CCD)                         src/core.jl:29-37
+ *     logpdf / logpdf!(::WhaleModel, ::Vector{<:CCD})      src/core.jl:46-64
+ *     (m::WhaleModel)(θ) reparameterisation                src/model.jl:147-160
+ *     read_ale(path, wm) / CCD(ale, wm, spmap)             src/ccd.jl:100-139
+ * A Julia front binds the functions below with `ccall`; the Python front in
+ * whale.jl_amd/ binds them with ctypes.  Each function names the reference
+ * code it replaces.
+ *
+ * Conventions
+ *   - every function returns 0 (WHALE_OK) or a WHALE_E* code; the message of
+ *     the last failure on the calling thread is whale_last_error();
+ *   - node arrays are indexed by reference node id - 1 (ids as assigned by
+ *     WhaleModel: leaves first, then the other non-WGD nodes in postorder, then
+ *     WGD nodes; src/model.jl:121-139);
+ *   - host buffers are owned by the caller; device memory is owned by the
+ *     handles; calls are synchronous (they return after the device work);
+ *   - a zero likelihood yields -Inf, never an error (src/core.jl:15,36);
+ *   - a context (and everything created from it) must be used by one host
+ *     thread at a time.
+ */
+#ifndef WHALE_AMD_H
+#define WHALE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  WHALE_OK = 0,
+  WHALE_EINVAL = 1,  /* bad argument or inconsistent input            */
+  WHALE_EPARSE = 2,  /* malformed ALE text (src/ccd.jl:150 assertion) */
+  WHALE_EHIP = 3,    /* HIP runtime failure                           */
+  WHALE_ENOMEM = 4,  /* device or host allocation failed              */
+  WHALE_ESTATE = 5   /* call made in the wrong state                  */
+};
+
+/* SamplingCondition (src/condition.jl:1-29) */
+enum { WHALE_COND_ROOT = 0, WHALE_COND_NONEXTINCT = 1, WHALE_COND_NONE = 2 };
+
+/* node kinds (src/model.jl:54 iswgd; NewickTree isleaf/isroot) */
+enum { WHALE_NODE_LEAF = 0, WHALE_NODE_SPECIATION = 1, WHALE_NODE_WGD = 2, WHALE_NODE_ROOT = 3 };
+
+/* arithmetic of the DP */
+enum {
+  WHALE_FP64_PLAIN = 0,   /* Float64 as the reference: underflows to -Inf like it      */
+  WHALE_FP32_SCALED = 1,  /* FP32 ℓ with per-clade power-of-two exponents              */
+  WHALE_FP64_SCALED = 2   /* FP64 ℓ with per-clade exponents (no underflow)            */
+};
+
+/* how a batch is launched */
+enum { WHALE_LAUNCH_BATCHED = 0, WHALE_LAUNCH_PER_FAMILY = 1 };
+
+typedef struct whale_ctx whale_ctx;
+typedef struct whale_model whale_model;
+typedef struct whale_ccd whale_ccd;
+typedef struct whale_batch whale_batch;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* whale_last_error(void);
+
+/* One HIP device, one stream.  No reference counterpart (the reference runs
+ * on Julia threads, src/core.jl:49). */
+int whale_ctx_create(int device, whale_ctx** out);
+int whale_ctx_destroy(whale_ctx* ctx);
+/* The hipStream_t the context launches on (for event timing by callers). */
+void* whale_ctx_stream(whale_ctx* ctx);
+
+/* Species-tree model.  Replaces WhaleModel(rates, Ψ, Δt; minn, maxn,
+ * condition) (src/model.jl:96-145): the caller has already discretised the
+ * tree (n_slices = k_e, slice_len = t_e/k_e as src/model.jl:16-22) and
+ * numbered it as the reference does.
+ *   parent[i]       parent id-1, or -1 for the root
+ *   kind[i]         WHALE_NODE_*
+ *   child0/child1   child id-1 (speciation/root: both; WGD: child0; else -1)
+ *   n_slices[i]     k_e >= 0 (rows k_e + 1)
+ *   slice_len[i]    Δ of the slices of the branch above the node
+ *   leaf_prob[i]    leafℙ (1, or 1/m in a MUL group; src/model.jl:110-111)
+ *   wgd_index[i]    0-based index into q for WGD nodes (wgdid - 1), else -1
+ *   leaf_species[i] species index in [0, n_species) for leaves, else -1
+ *                   (the clade id of setclade!, src/model.jl:32-42)
+ *   order           the n_nodes ids-1 of wm.order (children before parents)
+ */
+int whale_model_create(whale_ctx* ctx, int32_t n_nodes, const int32_t* parent, const int32_t* kind,
+                       const int32_t* child0, const int32_t* child1, const int32_t* n_slices,
+                       const double* slice_len, const double* leaf_prob, const int32_t* wgd_index,
+                       const int32_t* leaf_species, int32_t n_species, const int32_t* order,
+                       int32_t condition, whale_model** out);
+int whale_model_destroy(whale_model* m);
+
+/* Parameters on the natural scale, per node: the getθ of src/rmodels.jl:
+ * 31-33, 55-64 already applied by the caller (WGD nodes carry their
+ * non-WGD child's λ, μ; NaN where the reference has NaN).  q[w] per WGD
+ * index, p[i] sampling probability of leaf i (ignored elsewhere).  Builds the
+ * slice tables on the device (setmodel!, src/model.jl:162-191). */
+int whale_model_set_params(whale_model* m, const double* lambda, const double* mu, const double* q,
+                           int32_t n_q, double eta, const double* p);
+
+/* Slice-table layout and contents (for inspection): node i's rows are
+ * [off[i], off[i+1]) of eps/phi/psi (columns 2-4 of Slices.slices). */
+int whale_model_table_offsets(const whale_model* m, int32_t* off);
+int whale_model_tables(whale_model* m, double* eps, double* phi, double* psi);
+
+/* ALE v0.4 ingest: parse_aleobserve + CCD(ale, wm, spmap)
+ * (src/ccd.jl:102-121, 147-248) for one file's text.  species_names[k] maps
+ * to species index species_ids[k] (spmap of src/ccd.jl:128). */
+int whale_ccd_parse(const char* text, size_t len, const char* const* species_names,
+                    const int32_t* species_ids, int32_t n_names, whale_ccd** out);
+int whale_ccd_shape(const whale_ccd* c, int32_t* n_clades, int32_t* n_splits);
+/* Clades in the reference's order (by size, then ALE set id; 0-based ids).
+ * clade_size[n_clades], leaf_species[n_clades] (-1 for non-leaf clades),
+ * split_off[n_clades + 1], g1/g2/p[n_splits] (0-based clade ids). */
+int whale_ccd_export(const whale_ccd* c, int32_t* clade_size, int32_t* leaf_species, int32_t* split_off,
+                     int32_t* g1, int32_t* g2, double* p);
+int whale_ccd_destroy(whale_ccd* c);
+
+/* Device-resident batch of families (the Vector{CCD} of src/core.jl:46-64),
+ * flattened: family f owns clades [fam_clade_off[f], fam_clade_off[f+1]);
+ * clade c owns splits [split_off[c], split_off[c+1]) of g1/g2/p, with g1/g2
+ * family-local clade ids.  Within a family clades must be sorted by size,
+ * leaf clades (size 1) carry their species in leaf_species, the last clade is
+ * the ubiquitous one, and every split's parts are earlier clades.  Builds the
+ * per-family execution plan (compatible clades per branch, stage schedule,
+ * split and inverse-split lists) and uploads it. */
+int whale_batch_create(whale_model* m, int32_t n_families, const int64_t* fam_clade_off,
+                       const int32_t* clade_size, const int32_t* leaf_species, const int64_t* split_off,
+                       const int32_t* g1, const int32_t* g2, const double* p, whale_batch** out);
+int whale_batch_destroy(whale_batch* b);
+/* Workload counts: [families, clades, splits, cells, split_terms, sum_compat,
+ * items, records, stages, row_steps, max_stage_items, ell_cells]. */
+int whale_batch_stats(const whale_batch* b, int64_t* out12);
+
+/* logpdf of every family (unconditioned, src/core.jl:31-43) into
+ * out_family[n_families] (may be NULL) and the conditioned batch value
+ * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */
+int whale_logpdf(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total);
+
+/* logpdf plus its gradient with respect to the natural-scale per-node rates
+ * (d_lambda, d_mu: n_nodes), q (d_q: n_q) and η (d_eta), by a reverse sweep
+ * of the same DP.  The reference obtains this with ForwardDiff
+ * (test/runtests.jl:36-38); the branch of getα taken at |λ−μ| ≤ 1e-6 is the
+ * one differentiated, as ForwardDiff does. */
+int whale_logpdf_grad(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total,
+                      double* d_lambda, double* d_mu, double* d_q, double* d_eta);
+
+/* ℓ of one family from the last whale_logpdf/whale_logpdf_grad call, in the
+ * reference layout (logpdf! leaves it in CCD.ℓ, src/core.jl:29, for
+ * backtracking, src/track.jl:188-194): for node ids 1..n in turn, the
+ * (k_e+1) × #compat column-major matrix ℓ[e].  Values are unscaled doubles.
+ * *n_out receives the number of values; nothing is written if cap is too
+ * small. */
+int whale_download_ell(whale_batch* b, int32_t family, double* out, int64_t cap, int64_t* n_out);
+
+/* Per-kernel device time of the last call (ms), measured with HIP events on
+ * the context's stream when timing is enabled: [tables, forward, backward,
+ * gradient tail, total].  No reference counterpart. */
+int whale_ctx_set_timing(whale_ctx* ctx, int32_t enabled);
+int whale_last_timings(whale_ctx* ctx, double* ms5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WHALE_AMD_H */

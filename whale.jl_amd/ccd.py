@@ -1,0 +1,157 @@
+"""CCDs and read_ale, mirroring src/ccd.jl.  Parsing is done by the native
+library (whale_ccd_parse); a CCD holds the flattened clades in the
+reference's order (sorted by size then ALE set id; leaves first, the
+ubiquitous clade last), with 0-based clade ids."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, i32p, i64p, dblp, ptr
+
+
+class CCD:
+    __slots__ = ("size", "leaf_species", "split_off", "g1", "g2", "p", "fname", "ell", "_one", "__weakref__")
+
+    def __init__(self, size, leaf_species, split_off, g1, g2, p, fname: str = ""):
+        self.size = np.asarray(size, np.int32)
+        self.leaf_species = np.asarray(leaf_species, np.int32)
+        self.split_off = np.asarray(split_off, np.int64)
+        self.g1 = np.asarray(g1, np.int32)
+        self.g2 = np.asarray(g2, np.int32)
+        self.p = np.asarray(p, float)
+        self.fname = fname
+        self.ell = None  # per node id: (k+1) x |compat| matrix, set by logpdf_
+        self._one = None
+
+    def __len__(self):
+        return len(self.size)
+
+    @property
+    def nsplits(self) -> int:
+        return len(self.g1)
+
+    def __repr__(self):
+        return f"CCD(Γ={len(self)}, 𝓛={int(np.sum(self.size == 1))})"
+
+    @staticmethod
+    def from_text(text: str, spmap, fname: str = "") -> "CCD":
+        names = list(spmap.keys())
+        ids = np.array([spmap[k] - 1 for k in names], np.int32)
+        arr = (C.c_char_p * len(names))(*[k.encode() for k in names])
+        raw = text.encode()
+        h = C.c_void_p()
+        check(_lib.lib().whale_ccd_parse(raw, len(raw), arr, ptr(ids, i32p), len(names), C.byref(h)),
+              f"read_ale {fname}")
+        try:
+            nc, ns = C.c_int32(), C.c_int32()
+            check(_lib.lib().whale_ccd_shape(h, C.byref(nc), C.byref(ns)), "whale_ccd_shape")
+            size = np.zeros(nc.value, np.int32)
+            sp = np.zeros(nc.value, np.int32)
+            off = np.zeros(nc.value + 1, np.int32)
+            g1 = np.zeros(ns.value, np.int32)
+            g2 = np.zeros(ns.value, np.int32)
+            p = np.zeros(ns.value)
+            check(_lib.lib().whale_ccd_export(h, ptr(size, i32p), ptr(sp, i32p), ptr(off, i32p), ptr(g1, i32p),
+                                              ptr(g2, i32p), ptr(p, dblp)), "whale_ccd_export")
+        finally:
+            _lib.lib().whale_ccd_destroy(h)
+        return CCD(size, sp, off, g1, g2, p, fname)
+
+
+class CCDSet(list):
+    """A list of CCDs (what read_ale returns) that keeps its device batches."""
+
+    def __init__(self, it=()):
+        super().__init__(it)
+        self._batches = {}
+
+
+def read_ale(path: str, wm) -> CCDSet:
+    """read_ale(path, wm) (src/ccd.jl:126-137): one .ale file, a directory of
+    them (sorted), or a text file listing them."""
+    if not os.path.exists(path):
+        raise ValueError(f"Not a file nor directory `{path}`")
+    spmap = wm.s.spmap
+    if os.path.isfile(path) and path.endswith(".ale"):
+        files = [path]
+    else:
+        if os.path.isfile(path):
+            files = [l.rstrip("\n") for l in open(path)]
+        else:
+            files = [os.path.join(path, f) for f in sorted(os.listdir(path))]
+        files = [f for f in files if not f.startswith("#")]
+    out = CCDSet()
+    for f in files:
+        with open(f) as fh:
+            out.append(CCD.from_text(fh.read(), spmap, os.path.basename(f)))
+    return out
+
+
+def ccds_from_texts(texts: Sequence[str], wm, names: Optional[Sequence[str]] = None) -> CCDSet:
+    out = CCDSet()
+    for i, t in enumerate(texts):
+        out.append(CCD.from_text(t, wm.s.spmap, names[i] if names else f"family{i}"))
+    return out
+
+
+class DeviceBatch:
+    """A device-resident batch of families for one model structure
+    (whale_batch_create)."""
+
+    def __init__(self, wm, ccds: Sequence[CCD]):
+        self.s = wm.s
+        self.n = len(ccds)
+        nc = np.array([len(c) for c in ccds], np.int64)
+        fam_off = np.zeros(self.n + 1, np.int64)
+        np.cumsum(nc, out=fam_off[1:])
+        size = np.concatenate([c.size for c in ccds]).astype(np.int32)
+        sp = np.concatenate([c.leaf_species for c in ccds]).astype(np.int32)
+        g1 = np.concatenate([c.g1 for c in ccds]).astype(np.int32)
+        g2 = np.concatenate([c.g2 for c in ccds]).astype(np.int32)
+        p = np.concatenate([c.p for c in ccds]).astype(float)
+        offs = []
+        base = 0
+        for c in ccds:
+            offs.append(c.split_off[:-1].astype(np.int64) + base)
+            base += c.nsplits
+        split_off = np.concatenate(offs + [np.array([base], np.int64)])
+        h = C.c_void_p()
+        check(_lib.lib().whale_batch_create(wm.s.handle(), self.n, ptr(fam_off, i64p), ptr(size, i32p),
+                                            ptr(sp, i32p), ptr(split_off, i64p), ptr(g1, i32p), ptr(g2, i32p),
+                                            ptr(p, dblp), C.byref(h)), "whale_batch_create")
+        self.h = h
+
+    def stats(self) -> dict:
+        o = np.zeros(12, np.int64)
+        check(_lib.lib().whale_batch_stats(self.h, ptr(o, i64p)), "whale_batch_stats")
+        keys = ["families", "clades", "splits", "cells", "split_terms", "sum_compat", "items", "records", "stages",
+                "row_steps", "max_stage_items", "ell_cells"]
+        return dict(zip(keys, (int(x) for x in o)))
+
+    def ell(self, fam: int) -> np.ndarray:
+        n = C.c_int64()
+        check(_lib.lib().whale_download_ell(self.h, fam, None, 0, C.byref(n)), "whale_download_ell")
+        out = np.zeros(n.value)
+        check(_lib.lib().whale_download_ell(self.h, fam, ptr(out, dblp), n.value, C.byref(n)), "whale_download_ell")
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib._lib is not None:
+            _lib.lib().whale_batch_destroy(self.h)
+            self.h = None
+
+
+def device_batch(wm, ccds) -> DeviceBatch:
+    """The cached device batch of `ccds` for `wm`'s structure."""
+    if isinstance(ccds, CCDSet):
+        b = ccds._batches.get(id(wm.s))
+        if b is None or b.s is not wm.s:
+            b = DeviceBatch(wm, ccds)
+            ccds._batches[id(wm.s)] = b
+        return b
+    return DeviceBatch(wm, ccds)

@@ -1,0 +1,674 @@
+// C ABI (include/whale_amd.h): contexts, models, CCD ingest, device batches
+// and the evaluation entry points.  Host code; kernels live in kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/whale_amd.h"
+#include "dev.hpp"
+#include "host.hpp"
+
+using namespace whale;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(WHALE_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// device buffer owned by a handle
+struct DBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) n = bytes;
+    return e;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <typename T>
+hipError_t upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
+  hipError_t e = b.alloc(v.size() * sizeof(T));
+  if (e != hipSuccess || v.empty()) return e;
+  e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(st);
+}
+
+}  // namespace
+
+struct whale_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_cu = 256;
+  bool timing = false;
+  hipEvent_t ev[6] = {};
+  double last_ms[5] = {0, 0, 0, 0, 0};
+};
+
+struct whale_model {
+  whale_ctx* ctx = nullptr;
+  ModelHost h;
+  DBuf nodes, eps, phi, psi, drow, dnode, par, hlv, hnd, dlv, dnd, ebar, out;
+  int n_hlevels = 0, n_dlevels = 0;
+  bool has_params = false;
+  std::vector<double> host_par;
+  ModelDev dev() const {
+    ModelDev M;
+    M.node = nodes.as<NodeDev>();
+    M.eps = eps.as<double>();
+    M.phi = phi.as<double>();
+    M.psi = psi.as<double>();
+    M.par = par.as<double>();
+    M.n = h.n;
+    M.n_wgd = h.n_wgd;
+    M.root = h.root;
+    M.tab_rows = h.tab_rows;
+    return M;
+  }
+};
+
+struct whale_ccd {
+  CCDHost c;
+};
+
+struct whale_batch {
+  whale_model* model = nullptr;
+  int device = 0;
+  PlanHost plan;
+  DBuf fam, stage, seg, it_seg, it_gid, it_rec, it_inv, it_cf, it_cg, it_bar, rec_ops, rec_sid, inv_ops, inv_rec,
+      rec_p, clade_size, split_p, lvl, order;
+  // workspace
+  DBuf ell, coef, E, ibar, fam_ll, acc, ticket;
+  uint32_t n_fam = 0, n_large = 0;
+  uint32_t cap_small = 0, cap_large = 0;
+  int last_mode = -1;
+  bool last_store_all = false;
+  std::vector<uint32_t> order_host;
+  BatchDev dev() const {
+    BatchDev B;
+    B.fam = fam.as<FamDev>();
+    B.stage = stage.as<StageDev>();
+    B.seg = seg.as<SegDev>();
+    B.it_seg = it_seg.as<uint32_t>();
+    B.it_gid = it_gid.as<uint32_t>();
+    B.it_rec = it_rec.as<uint32_t>();
+    B.it_inv = it_inv.as<uint32_t>();
+    B.it_cf = it_cf.as<int32_t>();
+    B.it_cg = it_cg.as<int32_t>();
+    B.it_bar = it_bar.as<int32_t>();
+    B.rec_ops = rec_ops.as<uint32_t>();
+    B.rec_sid = rec_sid.as<uint32_t>();
+    B.inv_ops = inv_ops.as<uint32_t>();
+    B.inv_rec = inv_rec.as<uint32_t>();
+    B.rec_p = rec_p.as<double>();
+    B.clade_size = clade_size.as<uint16_t>();
+    B.split_p = split_p.as<double>();
+    B.lvl = lvl.as<uint32_t>();
+    B.order = order.as<uint32_t>();
+    B.n_fam = n_fam;
+    return B;
+  }
+  WorkRaw work() const {
+    WorkRaw W;
+    W.ell = ell.p;
+    W.coef = coef.p;
+    W.E = E.as<int32_t>();
+    W.ibar = ibar.p;
+    W.fam_ll = fam_ll.as<double>();
+    W.acc = acc.as<double>();
+    W.ticket = ticket.as<uint32_t>();
+    return W;
+  }
+};
+
+extern "C" {
+
+const char* whale_last_error(void) { return g_err.c_str(); }
+
+int whale_ctx_create(int device, whale_ctx** out) {
+  if (!out) return fail(WHALE_EINVAL, "out is NULL");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(WHALE_EINVAL, "no such HIP device");
+  HIPCHK(hipSetDevice(device));
+  auto c = std::make_unique<whale_ctx>();
+  c->device = device;
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  c->n_cu = prop.multiProcessorCount;
+  for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+  *out = c.release();
+  return WHALE_OK;
+}
+
+int whale_ctx_destroy(whale_ctx* c) {
+  if (!c) return WHALE_OK;
+  (void)hipSetDevice(c->device);
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return WHALE_OK;
+}
+
+void* whale_ctx_stream(whale_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int whale_ctx_set_timing(whale_ctx* c, int32_t en) {
+  if (!c) return fail(WHALE_EINVAL, "ctx is NULL");
+  c->timing = en != 0;
+  return WHALE_OK;
+}
+
+int whale_last_timings(whale_ctx* c, double* ms5) {
+  if (!c || !ms5) return fail(WHALE_EINVAL, "NULL argument");
+  for (int i = 0; i < 5; ++i) ms5[i] = c->last_ms[i];
+  return WHALE_OK;
+}
+
+int whale_model_create(whale_ctx* ctx, int32_t n, const int32_t* parent, const int32_t* kind, const int32_t* child0,
+                       const int32_t* child1, const int32_t* n_slices, const double* slice_len,
+                       const double* leaf_prob, const int32_t* wgd_index, const int32_t* leaf_species,
+                       int32_t n_species, const int32_t* order, int32_t condition, whale_model** out) {
+  if (!ctx || !out || n <= 0 || !parent || !kind || !child0 || !child1 || !n_slices || !slice_len || !leaf_prob ||
+      !wgd_index || !leaf_species || !order)
+    return fail(WHALE_EINVAL, "whale_model_create: NULL or empty argument");
+  if (condition < 0 || condition > 2) return fail(WHALE_EINVAL, "unknown condition");
+  if (n_species <= 0) return fail(WHALE_EINVAL, "n_species must be positive");
+  auto m = std::make_unique<whale_model>();
+  m->ctx = ctx;
+  ModelHost& h = m->h;
+  h.n = n;
+  h.n_species = n_species;
+  h.cond = condition;
+  h.words = (n_species + 63) / 64;
+  h.nodes.resize(n);
+  int nwgd = 0, roots = 0;
+  for (int i = 0; i < n; ++i) {
+    NodeDev& d = h.nodes[i];
+    d.kind = kind[i];
+    d.child0 = child0[i];
+    d.child1 = child1[i];
+    d.parent = parent[i];
+    d.R = n_slices[i] + 1;
+    d.wgd = wgd_index[i];
+    d.leafP = leaf_prob[i];
+    d.delta = slice_len[i];
+    d.pad = 0;
+    if (n_slices[i] < 0) return fail(WHALE_EINVAL, "negative slice count");
+    auto okc = [&](int c) { return c >= 0 && c < n && parent[c] == i; };
+    switch (d.kind) {
+      case WHALE_NODE_LEAF:
+        if (d.child0 != -1 || d.child1 != -1) return fail(WHALE_EINVAL, "leaf with children");
+        if (leaf_species[i] < 0 || leaf_species[i] >= n_species) return fail(WHALE_EINVAL, "leaf species out of range");
+        break;
+      case WHALE_NODE_WGD:
+        if (!okc(d.child0) || d.child1 != -1) return fail(WHALE_EINVAL, "WGD node needs exactly one child");
+        if (d.wgd < 0) return fail(WHALE_EINVAL, "WGD node without a q index");
+        nwgd = std::max(nwgd, d.wgd + 1);
+        break;
+      case WHALE_NODE_SPECIATION:
+      case WHALE_NODE_ROOT:
+        if (!okc(d.child0) || !okc(d.child1)) return fail(WHALE_EINVAL, "speciation/root node needs two children");
+        break;
+      default:
+        return fail(WHALE_EINVAL, "unknown node kind");
+    }
+    if (d.kind == WHALE_NODE_ROOT) { ++roots; h.root = i; if (parent[i] != -1) return fail(WHALE_EINVAL, "root with a parent"); }
+    else if (parent[i] < 0 || parent[i] >= n) return fail(WHALE_EINVAL, "non-root node without a parent");
+  }
+  if (roots != 1) return fail(WHALE_EINVAL, "the tree needs exactly one root");
+  h.n_wgd = nwgd;
+  // order must list every node once, children before parents
+  std::vector<int> pos(n, -1);
+  for (int k = 0; k < n; ++k) {
+    if (order[k] < 0 || order[k] >= n || pos[order[k]] >= 0) return fail(WHALE_EINVAL, "order is not a permutation");
+    pos[order[k]] = k;
+  }
+  for (int i = 0; i < n; ++i)
+    if (parent[i] >= 0 && pos[parent[i]] < pos[i]) return fail(WHALE_EINVAL, "order lists a parent before a child");
+  if (order[n - 1] != h.root) return fail(WHALE_EINVAL, "order must end with the root");
+  h.order.assign(order, order + n);
+  // species bitsets, heights, table offsets
+  h.bits.assign((size_t)n * h.words, 0);
+  h.height.assign(n, 0);
+  for (int e : h.order) {
+    const NodeDev& d = h.nodes[e];
+    uint64_t* b = &h.bits[(size_t)e * h.words];
+    if (d.kind == WHALE_NODE_LEAF) {
+      b[leaf_species[e] >> 6] |= 1ull << (leaf_species[e] & 63);
+    } else {
+      for (int c : {d.child0, d.child1}) {
+        if (c < 0) continue;
+        for (int w = 0; w < h.words; ++w) b[w] |= h.bits[(size_t)c * h.words + w];
+        h.height[e] = std::max(h.height[e], h.height[c] + 1);
+      }
+    }
+  }
+  h.max_height = h.height[h.root];
+  h.tab_off.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    h.nodes[i].tab = h.tab_off[i];
+    h.tab_off[i + 1] = h.tab_off[i] + h.nodes[i].R;
+  }
+  h.tab_rows = h.tab_off[n];
+  // height levels (children first) and depth levels (parents first)
+  std::vector<int32_t> hlv, hnd, dlv, dnd;
+  for (int L = 0; L <= h.max_height; ++L) {
+    hlv.push_back((int32_t)hnd.size());
+    for (int e : h.order) if (h.height[e] == L) hnd.push_back(e);
+  }
+  hlv.push_back((int32_t)hnd.size());
+  std::vector<int> depth(n, 0);
+  int maxd = 0;
+  for (int k = n - 1; k >= 0; --k) {
+    int e = h.order[k];
+    if (parent[e] >= 0) depth[e] = depth[parent[e]] + 1;
+    maxd = std::max(maxd, depth[e]);
+  }
+  for (int L = 0; L <= maxd; ++L) {
+    dlv.push_back((int32_t)dnd.size());
+    for (int e = 0; e < n; ++e) if (depth[e] == L) dnd.push_back(e);
+  }
+  dlv.push_back((int32_t)dnd.size());
+  m->n_hlevels = (int)hlv.size() - 1;
+  m->n_dlevels = (int)dlv.size() - 1;
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  HIPCHK(upload(m->nodes, h.nodes, st));
+  HIPCHK(upload(m->hlv, hlv, st));
+  HIPCHK(upload(m->hnd, hnd, st));
+  HIPCHK(upload(m->dlv, dlv, st));
+  HIPCHK(upload(m->dnd, dnd, st));
+  HIPCHK(m->eps.alloc(sizeof(double) * h.tab_rows));
+  HIPCHK(m->phi.alloc(sizeof(double) * h.tab_rows));
+  HIPCHK(m->psi.alloc(sizeof(double) * h.tab_rows));
+  HIPCHK(m->drow.alloc(sizeof(double) * 9 * h.tab_rows));
+  HIPCHK(m->dnode.alloc(sizeof(double) * 4 * n));
+  HIPCHK(m->par.alloc(sizeof(double) * (3 * n + nwgd + 1)));
+  HIPCHK(m->ebar.alloc(sizeof(double) * n));
+  HIPCHK(m->out.alloc(sizeof(double) * (2 + 2 * n + nwgd)));
+  *out = m.release();
+  return WHALE_OK;
+}
+
+int whale_model_destroy(whale_model* m) {
+  if (!m) return WHALE_OK;
+  (void)hipSetDevice(m->ctx->device);
+  delete m;
+  return WHALE_OK;
+}
+
+int whale_model_set_params(whale_model* m, const double* lambda, const double* mu, const double* q, int32_t n_q,
+                           double eta, const double* p) {
+  if (!m || !lambda || !mu) return fail(WHALE_EINVAL, "whale_model_set_params: NULL argument");
+  const int n = m->h.n;
+  if (n_q < m->h.n_wgd || (m->h.n_wgd > 0 && !q)) return fail(WHALE_EINVAL, "too few q values for the WGD nodes");
+  std::vector<double>& hp = m->host_par;
+  hp.assign(3 * n + m->h.n_wgd + 1, 0.0);
+  for (int i = 0; i < n; ++i) {
+    hp[i] = lambda[i];
+    hp[n + i] = mu[i];
+    hp[2 * n + i] = p ? p[i] : 0.0;
+  }
+  for (int w = 0; w < m->h.n_wgd; ++w) hp[3 * n + w] = q[w];
+  hp[3 * n + m->h.n_wgd] = eta;
+  whale_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(hipMemcpyAsync(m->par.p, hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_tables(c->stream, m->dev(), m->eps.as<double>(), m->phi.as<double>(), m->psi.as<double>(),
+                       m->drow.as<double>(), m->dnode.as<double>(), m->hlv.as<int32_t>(), m->hnd.as<int32_t>(),
+                       m->n_hlevels));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->timing) {
+    float ms;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_ms[0] = ms;
+  }
+  m->has_params = true;
+  return WHALE_OK;
+}
+
+int whale_model_table_offsets(const whale_model* m, int32_t* off) {
+  if (!m || !off) return fail(WHALE_EINVAL, "NULL argument");
+  for (int i = 0; i <= m->h.n; ++i) off[i] = m->h.tab_off[i];
+  return WHALE_OK;
+}
+
+int whale_model_tables(whale_model* m, double* eps, double* phi, double* psi) {
+  if (!m) return fail(WHALE_EINVAL, "NULL model");
+  if (!m->has_params) return fail(WHALE_ESTATE, "parameters not set");
+  HIPCHK(hipSetDevice(m->ctx->device));
+  size_t b = sizeof(double) * m->h.tab_rows;
+  if (eps) HIPCHK(hipMemcpy(eps, m->eps.p, b, hipMemcpyDeviceToHost));
+  if (phi) HIPCHK(hipMemcpy(phi, m->phi.p, b, hipMemcpyDeviceToHost));
+  if (psi) HIPCHK(hipMemcpy(psi, m->psi.p, b, hipMemcpyDeviceToHost));
+  return WHALE_OK;
+}
+
+int whale_ccd_parse(const char* text, size_t len, const char* const* names, const int32_t* ids, int32_t n_names,
+                    whale_ccd** out) {
+  if (!text || !out || (n_names > 0 && (!names || !ids))) return fail(WHALE_EINVAL, "whale_ccd_parse: NULL argument");
+  std::unordered_map<std::string, int> sp;
+  for (int k = 0; k < n_names; ++k) sp[names[k]] = ids[k];
+  auto c = std::make_unique<whale_ccd>();
+  std::string err = parse_ale(text, len, sp, c->c);
+  if (!err.empty()) return fail(WHALE_EPARSE, err);
+  *out = c.release();
+  return WHALE_OK;
+}
+
+int whale_ccd_shape(const whale_ccd* c, int32_t* nc, int32_t* ns) {
+  if (!c || !nc || !ns) return fail(WHALE_EINVAL, "NULL argument");
+  *nc = (int32_t)c->c.size.size();
+  *ns = (int32_t)c->c.g1.size();
+  return WHALE_OK;
+}
+
+int whale_ccd_export(const whale_ccd* c, int32_t* size, int32_t* leaf_species, int32_t* split_off, int32_t* g1,
+                     int32_t* g2, double* p) {
+  if (!c) return fail(WHALE_EINVAL, "NULL ccd");
+  const CCDHost& h = c->c;
+  if (size) std::copy(h.size.begin(), h.size.end(), size);
+  if (leaf_species) std::copy(h.leaf_species.begin(), h.leaf_species.end(), leaf_species);
+  if (split_off) std::copy(h.split_off.begin(), h.split_off.end(), split_off);
+  if (g1) std::copy(h.g1.begin(), h.g1.end(), g1);
+  if (g2) std::copy(h.g2.begin(), h.g2.end(), g2);
+  if (p) std::copy(h.p.begin(), h.p.end(), p);
+  return WHALE_OK;
+}
+
+int whale_ccd_destroy(whale_ccd* c) {
+  delete c;
+  return WHALE_OK;
+}
+
+int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_off, const int32_t* clade_size,
+                       const int32_t* leaf_species, const int64_t* split_off, const int32_t* g1, const int32_t* g2,
+                       const double* p, whale_batch** out) {
+  if (!m || !out || n_fam <= 0 || !fam_clade_off || !clade_size || !leaf_species || !split_off || !g1 || !g2 || !p)
+    return fail(WHALE_EINVAL, "whale_batch_create: NULL or empty argument");
+  if (fam_clade_off[0] != 0) return fail(WHALE_EINVAL, "fam_clade_off[0] must be 0");
+  for (int f = 0; f < n_fam; ++f)
+    if (fam_clade_off[f + 1] < fam_clade_off[f]) return fail(WHALE_EINVAL, "fam_clade_off is not increasing");
+  int64_t nc = fam_clade_off[n_fam];
+  for (int64_t c = 0; c < nc; ++c) {
+    if (split_off[c + 1] < split_off[c]) return fail(WHALE_EINVAL, "split_off is not increasing");
+    if (clade_size[c] < 1) return fail(WHALE_EINVAL, "clade size must be positive");
+  }
+  if (nc > 0xffffffffLL || split_off[nc] > 0xffffffffLL) return fail(WHALE_EINVAL, "batch too large");
+  auto b = std::make_unique<whale_batch>();
+  b->model = m;
+  b->device = m->ctx->device;
+  unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  std::string err = build_plan(m->h, n_fam, fam_clade_off, clade_size, leaf_species, split_off, g1, g2, p, b->plan,
+                               (int)std::min(hw, 16u));
+  if (!err.empty()) return fail(WHALE_EINVAL, err);
+  PlanHost& P = b->plan;
+  if (P.rec_ops.size() > 0xffffffffull || P.inv_ops.size() > 0xffffffffull || P.it_seg.size() > 0xffffffffull)
+    return fail(WHALE_EINVAL, "batch too large");
+  b->n_fam = (uint32_t)n_fam;
+  // size classes: families whose largest stage fits the small LDS cap run in a
+  // launch with many workgroups per CU; the rest get their own launch.
+  const uint32_t small_cap = 512;
+  std::vector<uint32_t> large, small;
+  for (uint32_t f : P.cost_order) (P.fam[f].max_items > small_cap ? large : small).push_back(f);
+  b->order_host = large;
+  b->order_host.insert(b->order_host.end(), small.begin(), small.end());
+  b->n_large = (uint32_t)large.size();
+  b->cap_small = small_cap;
+  b->cap_large = P.max_items;
+  whale_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  HIPCHK(upload(b->fam, P.fam, st));
+  HIPCHK(upload(b->stage, P.stage, st));
+  HIPCHK(upload(b->seg, P.seg, st));
+  HIPCHK(upload(b->it_seg, P.it_seg, st));
+  HIPCHK(upload(b->it_gid, P.it_gid, st));
+  HIPCHK(upload(b->it_rec, P.it_rec, st));
+  HIPCHK(upload(b->it_inv, P.it_inv, st));
+  HIPCHK(upload(b->it_cf, P.it_cf, st));
+  HIPCHK(upload(b->it_cg, P.it_cg, st));
+  HIPCHK(upload(b->it_bar, P.it_bar, st));
+  HIPCHK(upload(b->rec_ops, P.rec_ops, st));
+  HIPCHK(upload(b->rec_sid, P.rec_sid, st));
+  HIPCHK(upload(b->inv_ops, P.inv_ops, st));
+  HIPCHK(upload(b->inv_rec, P.inv_rec, st));
+  HIPCHK(upload(b->rec_p, P.rec_p, st));
+  HIPCHK(upload(b->clade_size, P.clade_size, st));
+  HIPCHK(upload(b->split_p, P.split_p, st));
+  HIPCHK(upload(b->lvl, P.lvl, st));
+  HIPCHK(upload(b->order, b->order_host, st));
+  HIPCHK(b->ell.alloc(sizeof(double) * std::max<uint64_t>(1, P.ell_cells)));
+  HIPCHK(b->coef.alloc(sizeof(double) * std::max<size_t>(1, P.rec_ops.size())));
+  HIPCHK(b->E.alloc(sizeof(int32_t) * std::max<int64_t>(1, nc)));
+  HIPCHK(b->ibar.alloc(sizeof(double) * std::max<uint64_t>(1, P.ibar_cells)));
+  HIPCHK(b->fam_ll.alloc(sizeof(double) * n_fam));
+  AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+  HIPCHK(b->acc.alloc(sizeof(double) * AL.n));
+  HIPCHK(b->ticket.alloc(sizeof(uint32_t) * 8));
+  // release host copies the device now owns (keep what ℓ download needs)
+  std::vector<uint32_t>().swap(P.it_rec);
+  std::vector<uint32_t>().swap(P.it_inv);
+  std::vector<int32_t>().swap(P.it_cf);
+  std::vector<int32_t>().swap(P.it_cg);
+  std::vector<int32_t>().swap(P.it_bar);
+  std::vector<uint32_t>().swap(P.rec_ops);
+  std::vector<uint32_t>().swap(P.rec_sid);
+  std::vector<uint32_t>().swap(P.inv_ops);
+  std::vector<uint32_t>().swap(P.inv_rec);
+  std::vector<double>().swap(P.rec_p);
+  *out = b.release();
+  return WHALE_OK;
+}
+
+int whale_batch_destroy(whale_batch* b) {
+  if (!b) return WHALE_OK;
+  (void)hipSetDevice(b->device);  // the model may already be gone
+  delete b;
+  return WHALE_OK;
+}
+
+int whale_batch_stats(const whale_batch* b, int64_t* o) {
+  if (!b || !o) return fail(WHALE_EINVAL, "NULL argument");
+  const PlanHost& P = b->plan;
+  o[0] = b->n_fam;
+  o[1] = P.n_clades;
+  o[2] = P.n_splits;
+  o[3] = P.cells;
+  o[4] = P.split_terms;
+  o[5] = P.sum_compat;
+  o[6] = (int64_t)P.it_seg.size();
+  o[7] = (int64_t)b->rec_ops.n / sizeof(uint32_t);
+  o[8] = (int64_t)P.stage.size();
+  o[9] = P.row_steps;
+  o[10] = P.max_items;
+  o[11] = (int64_t)P.ell_cells;
+  return WHALE_OK;
+}
+
+namespace {
+
+int grid_for(const whale_batch* b, int mode, uint32_t cap, uint32_t nfam) {
+  // workgroups resident per CU: LDS-limited, at most 16 one-wave workgroups
+  size_t lds = std::max<size_t>(1, lds_bytes(mode, cap));
+  int per_cu = (int)std::min<size_t>(16, std::max<size_t>(1, (160 * 1024) / lds));
+  int g = b->model->ctx->n_cu * per_cu;
+  return (int)std::min<uint32_t>((uint32_t)g, std::max<uint32_t>(1u, nfam));
+}
+
+int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_family, double* out_total,
+        double* d_lambda, double* d_mu, double* d_q, double* d_eta) {
+  if (!b || !out_total) return fail(WHALE_EINVAL, "NULL argument");
+  whale_model* m = b->model;
+  whale_ctx* c = m->ctx;
+  if (!m->has_params) return fail(WHALE_ESTATE, "parameters not set (whale_model_set_params)");
+  const int mode = mode_in & 0xff;
+  const bool keep = (mode_in & 0x100) != 0;
+  if (mode < 0 || mode > 2) return fail(WHALE_EINVAL, "unknown mode");
+  if (launch != WHALE_LAUNCH_BATCHED && launch != WHALE_LAUNCH_PER_FAMILY) return fail(WHALE_EINVAL, "unknown launch");
+  const bool store_all = grad || keep;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const BatchDev B = b->dev();
+  const ModelDev M = m->dev();
+  const WorkRaw W = b->work();
+  const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
+  HIPCHK(launch_fill_u32(st, W.ticket, 0u, 8));
+  if (grad) HIPCHK(launch_fill_f64(st, W.acc, 0.0, AL.n));
+  const uint32_t nl = b->n_large, nf = b->n_fam;
+  if (launch == WHALE_LAUNCH_BATCHED) {
+    if (nl) HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_large, nl), b->cap_large, st, B, M, W, -1, 0, nl));
+    if (nf > nl) {
+      WorkRaw W2 = W;
+      W2.ticket = W.ticket + 1;
+      HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_small, nf - nl), b->cap_small, st, B, M, W2, -1, nl, nf));
+    }
+  } else {
+    for (uint32_t k = 0; k < nf; ++k) {
+      uint32_t f = b->order_host[k];
+      uint32_t cap = b->plan.fam[f].max_items;
+      HIPCHK(launch_forward(mode, store_all, 1, cap, st, B, M, W, (int32_t)f, 0, 0));
+    }
+  }
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[2], st));
+  if (grad) {
+    if (launch == WHALE_LAUNCH_BATCHED) {
+      WorkRaw W3 = W;
+      W3.ticket = W.ticket + 2;
+      if (nl) HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_large, nl), b->cap_large, st, B, M, W3, -1, 0, nl));
+      if (nf > nl) {
+        W3.ticket = W.ticket + 3;
+        HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_small, nf - nl), b->cap_small, st, B, M, W3, -1, nl, nf));
+      }
+    } else {
+      for (uint32_t k = 0; k < nf; ++k) {
+        uint32_t f = b->order_host[k];
+        uint32_t cap = b->plan.fam[f].max_items;
+        HIPCHK(launch_backward(mode, 1, cap, st, B, M, W, (int32_t)f, 0, 0));
+      }
+    }
+  }
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[3], st));
+  double* out = m->out.as<double>();
+  HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
+                     m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[4], st));
+  const int n = m->h.n, nw = m->h.n_wgd;
+  std::vector<double> o(2 + 2 * n + nw);
+  HIPCHK(hipMemcpyAsync(o.data(), out, sizeof(double) * (grad ? o.size() : 1), hipMemcpyDeviceToHost, st));
+  if (out_family) HIPCHK(hipMemcpyAsync(out_family, W.fam_ll, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (c->timing) {
+    float a, bb, cc, dd;
+    HIPCHK(hipEventElapsedTime(&a, c->ev[1], c->ev[2]));
+    HIPCHK(hipEventElapsedTime(&bb, c->ev[2], c->ev[3]));
+    HIPCHK(hipEventElapsedTime(&cc, c->ev[3], c->ev[4]));
+    HIPCHK(hipEventElapsedTime(&dd, c->ev[1], c->ev[4]));
+    c->last_ms[1] = a;
+    c->last_ms[2] = bb;
+    c->last_ms[3] = cc;
+    c->last_ms[4] = dd;
+  }
+  *out_total = o[0];
+  if (grad) {
+    if (d_lambda) std::copy(o.begin() + 1, o.begin() + 1 + n, d_lambda);
+    if (d_mu) std::copy(o.begin() + 1 + n, o.begin() + 1 + 2 * n, d_mu);
+    if (d_q) std::copy(o.begin() + 1 + 2 * n, o.begin() + 1 + 2 * n + nw, d_q);
+    if (d_eta) *d_eta = o[1 + 2 * n + nw];
+  }
+  b->last_mode = mode;
+  b->last_store_all = store_all;
+  return WHALE_OK;
+}
+
+}  // namespace
+
+int whale_logpdf(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total) {
+  return run(b, mode, launch, false, out_family, out_total, nullptr, nullptr, nullptr, nullptr);
+}
+
+int whale_logpdf_grad(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total,
+                      double* d_lambda, double* d_mu, double* d_q, double* d_eta) {
+  return run(b, mode, launch, true, out_family, out_total, d_lambda, d_mu, d_q, d_eta);
+}
+
+int whale_download_ell(whale_batch* b, int32_t f, double* out, int64_t cap, int64_t* n_out) {
+  if (!b || !n_out) return fail(WHALE_EINVAL, "NULL argument");
+  if (f < 0 || (uint32_t)f >= b->n_fam) return fail(WHALE_EINVAL, "family out of range");
+  if (b->last_mode < 0 || !b->last_store_all)
+    return fail(WHALE_ESTATE, "no stored ℓ: evaluate with WHALE_KEEP_ELL or with the gradient first");
+  const whale_model* m = b->model;
+  const PlanHost& P = b->plan;
+  const FamDev& F = P.fam[f];
+  // reference layout sizes: per node id, (k+1) x |compat|
+  std::vector<int> segof(m->h.n, -1);
+  for (uint32_t s = P.fam_seg0[f]; s < P.fam_seg0[f] + P.fam_nseg[f]; ++s) segof[P.seg[s].node] = (int)s;
+  int64_t total = 0;
+  for (int e = 0; e < m->h.n; ++e)
+    if (segof[e] >= 0) total += (int64_t)m->h.nodes[e].R * P.seg[segof[e]].nc;
+  *n_out = total;
+  if (!out || cap < total) return WHALE_OK;
+  const int mode = b->last_mode;
+  const size_t rs = mode == WHALE_FP32_SCALED ? 4 : 8;
+  uint64_t cells = (f + 1 < (int32_t)b->n_fam ? P.fam[f + 1].ell0 : P.ell_cells) - F.ell0;
+  std::vector<unsigned char> raw(cells * rs);
+  std::vector<int32_t> E(F.n_clades, 0);
+  HIPCHK(hipSetDevice(m->ctx->device));
+  HIPCHK(hipMemcpy(raw.data(), (char*)b->ell.p + F.ell0 * rs, raw.size(), hipMemcpyDeviceToHost));
+  if (mode != WHALE_FP64_PLAIN)
+    HIPCHK(hipMemcpy(E.data(), b->E.as<int32_t>() + F.clade0, sizeof(int32_t) * F.n_clades, hipMemcpyDeviceToHost));
+  // compatible clades of each segment, in increasing clade id (the same order as the reference's compat)
+  int64_t k = 0;
+  for (int e = 0; e < m->h.n; ++e) {
+    int s = segof[e];
+    if (s < 0) continue;
+    const SegDev& sg = P.seg[s];
+    // items of segment s are contiguous and it_seg is non-decreasing; they carry the clade ids
+    std::vector<uint32_t> cl(sg.nc);
+    uint32_t it0 = (uint32_t)(std::lower_bound(P.it_seg.begin(), P.it_seg.end(), (uint32_t)s) - P.it_seg.begin());
+    for (uint32_t j = 0; j < sg.nc; ++j) cl[j] = P.it_gid[it0 + j];
+    const int R = m->h.nodes[e].R;
+    for (uint32_t j = 0; j < sg.nc; ++j) {
+      for (int r = 0; r < R; ++r) {
+        double v = 0.0;
+        if (r < (int)sg.R) {
+          size_t cell = sg.ell + (size_t)r * sg.nc + j;
+          if (rs == 4) v = (double)reinterpret_cast<const float*>(raw.data())[cell];
+          else v = reinterpret_cast<const double*>(raw.data())[cell];
+          if (mode != WHALE_FP64_PLAIN && v != 0.0) v = std::ldexp(v, E[cl[j]]);
+        }
+        out[k++] = v;
+      }
+    }
+  }
+  return WHALE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Device-side views of the model, the batch plan and the per-call workspace,
+// and the launch wrappers the C ABI calls.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "plan.hpp"
+
+namespace whale {
+
+struct BatchDev {
+  const FamDev* fam;
+  const StageDev* stage;
+  const SegDev* seg;
+  const uint32_t* it_seg;
+  const uint32_t* it_gid;
+  const uint32_t* it_rec;
+  const uint32_t* it_inv;
+  const int32_t* it_cf;
+  const int32_t* it_cg;
+  const int32_t* it_bar;
+  const uint32_t* rec_ops;
+  const uint32_t* rec_sid;
+  const uint32_t* inv_ops;
+  const uint32_t* inv_rec;
+  const double* rec_p;
+  const uint16_t* clade_size;
+  const double* split_p;
+  const uint32_t* lvl;
+  const uint32_t* order;   // families in processing order
+  uint32_t n_fam;
+};
+
+struct ModelDev {
+  const NodeDev* node;
+  const double* eps;
+  const double* phi;
+  const double* psi;
+  const double* par;       // [λ(n), μ(n), p(n), q(n_wgd), η]
+  int n;
+  int n_wgd;
+  int root;
+  int tab_rows;
+};
+
+// type-erased workspace (ell, coef, ibar hold float or double)
+struct WorkRaw {
+  void* ell;
+  void* coef;
+  int32_t* E;
+  void* ibar;
+  double* fam_ll;
+  double* acc;
+  uint32_t* ticket;
+};
+
+// gradient accumulator layout (doubles)
+struct AccLayout {
+  int phi, psi, eps, q, rootA, rootB, n;
+  __host__ __device__ AccLayout(int tab_rows, int nodes, int nwgd) {
+    phi = 0;
+    psi = tab_rows;
+    eps = 2 * tab_rows;
+    q = eps + nodes;
+    rootA = q + nwgd;
+    rootB = rootA + 1;
+    n = rootB + 1;
+  }
+};
+
+// forward/backward launch: `mode` is WHALE_FP64_PLAIN / WHALE_FP32_SCALED /
+// WHALE_FP64_SCALED; fam_single >= 0 runs that one family on a 1-block grid,
+// otherwise blocks take families order[fam_begin .. fam_end) from ticket.
+hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, hipStream_t st, const BatchDev& B,
+                          const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
+                          uint32_t fam_end);
+hipError_t launch_backward(int mode, int grid, uint32_t cap, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                           const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end);
+hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
+                         double* dnode, const int32_t* hlevels, const int32_t* hnodes, int n_levels);
+hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,
+                       double* ebar, const double* drow, const double* dnode, const int32_t* dlevels,
+                       const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out);
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int n);
+hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n);
+size_t lds_bytes(int mode, uint32_t cap);
+
+}  // namespace whale
