@@ -1,0 +1,220 @@
+"""Benchmark of the amalgamated-likelihood DP on one MI355X.
+
+One step = one log-likelihood + gradient evaluation of the whole batch at a
+parameter point: slice tables (k_tables), forward DP of every family
+(k_forward), reverse sweep (k_backward) and the table adjoint / batch sum
+(k_tail).  The workload is the large-batch configuration (config4: 50-taxon
+species tree with 4 WGDs, Δt = 0.01, 10,000 families of 4–200 leaves), with
+synthetic, seeded data; inputs are device-resident before timing starts.
+
+Printed JSON line: families/s of batched FP32-scaled logL+grad, with its
+error against the FP64-scaled form, the FP64 plain form launched per family
+for comparison, the roofline of the dominant kernel and the CPU oracle's
+throughput on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config4]
+                    [--no-cpu] [--no-per-family] [--cpu-seconds S]
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(st, s_bytes, x_fwd=0, x_bwd=0):
+    """Algorithmic HBM bytes of one evaluation: each ℓ cell written once and
+    read once in the forward (2·s per cell), 16 B per split (two int32 ids and
+    an fp64 p), 4 B per compatible clade, the exponents X stored by the scaled
+    forms and the 8-byte result; the gradient adds the ℓ re-read, the adjoint
+    write and read (3·s per cell) and the split data again."""
+    fwd = 2 * s_bytes * st["cells"] + 16 * st["splits"] + 4 * st["sum_compat"] + x_fwd + 8
+    grad = fwd + 3 * s_bytes * st["cells"] + 16 * st["splits"] + x_bwd
+    return fwd, grad
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="config4")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-per-family", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--families", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (families would be sharded across "
+              "GPUs with one all-reduce of the gradient; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from __graft_entry__ import load_package
+    w = load_package()
+    from whale_jl_amd import _lib, synth
+    from whale_jl_amd._lib import check, dblp, ptr
+    L = _lib.lib()
+
+    t0 = time.time()
+    wl = synth.workload(args.workload, n_families=args.families)
+    log(f"workload {args.workload}: {len(wl.texts)} families generated in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    model = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt)
+    ccds = w.ccds_from_texts(wl.texts, model)
+    batch = w.device_batch(model, ccds)
+    st = batch.stats()
+    log(f"parsed and planned in {time.time() - t0:.1f}s: {st}")
+    n_fam = st["families"]
+    ctx = w.context()
+    h = model.s.handle()
+    lam, mu, q, eta, p, _ = model.node_params()
+    q = np.ascontiguousarray(q if len(q) else np.zeros(1))
+    nn, nw = model.s.n, model.s.n_wgd
+    fam = np.zeros(n_fam)
+    dl, dm, dq = np.zeros(nn), np.zeros(nn), np.zeros(max(1, nw))
+    deta, tot = C.c_double(), C.c_double()
+
+    def step(mode, launch=_lib.BATCHED, grad=True):
+        check(L.whale_model_set_params(h, ptr(lam, dblp), ptr(mu, dblp), ptr(q, dblp), len(q), eta, ptr(p, dblp)),
+              "set_params")
+        if grad:
+            check(L.whale_logpdf_grad(batch.h, mode, launch, ptr(fam, dblp), C.byref(tot), ptr(dl, dblp),
+                                      ptr(dm, dblp), ptr(dq, dblp), C.byref(deta)), "logpdf_grad")
+        else:
+            check(L.whale_logpdf(batch.h, mode, launch, ptr(fam, dblp), C.byref(tot)), "logpdf")
+        return tot.value, fam.copy(), np.concatenate([dl, dm, dq[:nw], [deta.value]])
+
+    # ---- accuracy: FP32-scaled vs FP64-scaled (no underflow) and FP64 plain
+    v64s, f64s, g64s = step(_lib.FP64_SCALED)
+    v64, f64, g64 = step(_lib.FP64_PLAIN)
+    v32, f32, g32 = step(_lib.FP32_SCALED)
+    fin = np.isfinite(f64s)
+    rel_fam = float(np.max(np.abs(f32[fin] - f64s[fin]) / np.abs(f64s[fin])))
+    rel_tot = abs(v32 - v64s) / abs(v64s)
+    gerr = float(np.max(np.abs(g32 - g64s)) / np.max(np.abs(g64s)))
+    n_underflow = int(np.sum(~np.isfinite(f64)))
+    log(f"logL fp64s {v64s:.6f} fp64 {v64:.6f} fp32 {v32:.6f}; fp32 rel err family max {rel_fam:.2e} "
+        f"total {rel_tot:.2e} grad {gerr:.2e}; fp64-plain underflowed families {n_underflow}")
+
+    # ---- timed: batched FP32-scaled logL + gradient
+    for _ in range(args.warmup):
+        step(_lib.FP32_SCALED)
+    L.whale_ctx_set_timing(ctx, 1)
+    ms5 = np.zeros(5)
+    kern = np.zeros(5)
+    # HIP events on the library's stream bracket the timed steps
+    t_wall = time.perf_counter()
+    check(L.whale_ctx_record(ctx, 0), "record")
+    for _ in range(args.steps):
+        step(_lib.FP32_SCALED)
+        L.whale_last_timings(ctx, ptr(ms5, dblp))
+        kern += ms5
+    check(L.whale_ctx_record(ctx, 1), "record")
+    ms_c = C.c_double()
+    check(L.whale_ctx_elapsed(ctx, 0, 1, C.byref(ms_c)), "elapsed")
+    wall = time.perf_counter() - t_wall
+    ms_total = ms_c.value
+    L.whale_ctx_set_timing(ctx, 0)
+    ms_step = ms_total / args.steps
+    kern /= args.steps  # [tables, forward, backward, tail, total]
+    value = n_fam / (ms_step / 1e3)
+    log(f"fp32-scaled batched: {ms_step:.3f} ms/step (wall {1e3 * wall / args.steps:.3f}), {value:.4g} families/s; "
+        f"kernels ms: tables {kern[0]:.3f} fwd {kern[1]:.3f} bwd {kern[2]:.3f} tail {kern[3]:.3f}")
+
+    # ---- FP64 plain batched and per-family launches (the comparison forms)
+    step(_lib.FP64_PLAIN)
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        step(_lib.FP64_PLAIN)
+    fp64_batched_ms = 1e3 * (time.perf_counter() - t) / reps
+    per_fam = None
+    if not args.no_per_family:
+        t = time.perf_counter()
+        step(_lib.FP64_PLAIN, launch=_lib.PER_FAMILY)
+        pf_s = time.perf_counter() - t
+        per_fam = n_fam / pf_s
+        log(f"fp64 plain per-family launches: {pf_s:.3f} s, {per_fam:.4g} families/s; "
+            f"fp64 plain batched {fp64_batched_ms:.3f} ms")
+
+    # ---- roofline of the dominant kernel (algorithmic bytes / its duration)
+    b_fwd, b_grad = algorithmic_bytes(st, 4, x_fwd=4 * st["clades"])
+    fwd_gbs = b_fwd / (kern[1] / 1e3) / 1e9
+    bwd_gbs = (b_grad - b_fwd) / (kern[2] / 1e3) / 1e9
+    if kern[2] >= kern[1]:
+        dom, ach, traffic_bytes = "k_backward", bwd_gbs, b_grad - b_fwd
+    else:
+        dom, ach, traffic_bytes = "k_forward", fwd_gbs, b_fwd
+    log(f"roofline: fwd {fwd_gbs:.1f} GB/s ({b_fwd / 1e9:.3f} GB), bwd {bwd_gbs:.1f} GB/s "
+        f"({(b_grad - b_fwd) / 1e9:.3f} GB), whole step {b_grad / (ms_step / 1e3) / 1e9:.1f} GB/s")
+
+    # ---- CPU baseline: the oracle (C++ restatement) on a bounded sample
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import OracleModel, DLWGD as ODLWGD
+        ncpu = min(16, os.cpu_count() or 1)
+        om = OracleModel(wl.newick, [], wl.dt)
+        r = wl.rates
+        orr = ODLWGD(list(r.λ), list(r.μ), list(r.q), r.η)
+        # sample every k-th family, growing until the gradient pass takes about cpu_seconds
+        k = max(1, n_fam // 8)
+        while True:
+            idx = list(range(0, n_fam, k))
+            oc = om.ccds([wl.texts[i] for i in idx])
+            t = time.perf_counter()
+            om.gradient(oc, orr, nthreads=ncpu)
+            dt = time.perf_counter() - t
+            if dt * 2.5 > args.cpu_seconds or k == 1:
+                break
+            k = max(1, int(k * dt / args.cpu_seconds * 2) or 1)
+        cpu = dict(value=len(idx) / dt, unit="families/s", cores=ncpu, kind="port",
+                   sample=f"logL+ForwardDiff-style gradient (chunks of 12 duals) of {len(idx)} of the {n_fam} "
+                          f"families (every {k}th), oracle C++ FP64 restatement, {dt:.1f} s on {ncpu} threads")
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "families/s for logL+grad: FP32-scaled batched vs FP64 plain, rel err ≤1e-6",
+        "value": value,
+        "unit": "families/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {n_fam} families, 50-taxon tree + 4 WGDs, dt=0.01, "
+                               "logL+gradient, FP32 with per-clade exponents, one batched launch",
+                   "families": n_fam, "cells": st["cells"], "split_terms": st["split_terms"]},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "algorithmic_bytes": traffic_bytes},
+        "cpu_baseline": cpu,
+        "accuracy": {"fp32_vs_fp64_family_max_rel": rel_fam, "fp32_vs_fp64_total_rel": rel_tot,
+                     "fp32_vs_fp64_grad_rel": gerr, "fp64_plain_underflowed_families": n_underflow},
+        "fp64_plain_per_family_families_per_s": per_fam,
+        "speedup_vs_fp64_per_family": (value / per_fam) if per_fam else None,
+        "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "fp64_plain_batched_ms": fp64_batched_ms,
+        "kernel_ms": {"tables": kern[0], "forward": kern[1], "backward": kern[2], "tail": kern[3]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -159,6 +159,12 @@ int whale_logpdf_grad(whale_batch* b, int32_t mode, int32_t launch, double* out_
  * small. */
 int whale_download_ell(whale_batch* b, int32_t family, double* out, int64_t cap, int64_t* n_out);
 
+/* HIP events on the context's stream for callers' own timing: record event
+ * `slot` (0..15) now; elapsed ms between two recorded slots (synchronises on
+ * the later one).  No reference counterpart. */
+int whale_ctx_record(whale_ctx* ctx, int32_t slot);
+int whale_ctx_elapsed(whale_ctx* ctx, int32_t from_slot, int32_t to_slot, double* ms);
+
 /* Per-kernel device time of the last call (ms), measured with HIP events on
  * the context's stream when timing is enabled: [tables, forward, backward,
  * gradient tail, total].  No reference counterpart. */

@@ -1,0 +1,172 @@
+"""More HIP-path parity cases against the CPU oracle: WGD workloads with
+branch-wise rates, MUL trees, the three conditions, constant rates, the
+λ = μ branch of getα, ℓ download and the discretisation invariant."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import EXTREE, GOLDEN, KAT_INSERTS, MULTREE, example1_texts
+from oracle.oracle import OracleModel, DLWGD as ODLWGD, ConstantDLWGD as OConst
+
+pytestmark = pytest.mark.gpu
+
+
+def orates(r):
+    return ODLWGD(list(r.λ), list(r.μ), list(r.q), r.η, list(r.p))
+
+
+@pytest.fixture(scope="module")
+def cfg2(w):
+    from whale_jl_amd import synth
+    wl = synth.workload("config2", n_families=120)
+    om = OracleModel(wl.newick, [], wl.dt)
+    oc = om.ccds(wl.texts)
+    m = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt)
+    c = w.ccds_from_texts(wl.texts, m)
+    return wl, om, oc, m, c
+
+
+def test_config2_wgd_branch_rates_fp64(w, cfg2):
+    wl, om, oc, m, c = cfg2
+    ref = om.logpdf(oc, orates(wl.rates), per_family=True)
+    got = w.logpdf_families(m, c)
+    fin = np.isfinite(ref)
+    assert np.array_equal(fin, np.isfinite(got))
+    assert np.max(np.abs(got[fin] - ref[fin]) / np.abs(ref[fin])) <= 1e-12
+    assert abs(w.logpdf(m, c) - om.logpdf(oc, orates(wl.rates))) <= 1e-12 * abs(om.logpdf(oc, orates(wl.rates)))
+
+
+def test_config2_scaled_forms_vs_long_double(w, cfg2):
+    wl, om, oc, m, c = cfg2
+    ref = om.logpdf(oc, orates(wl.rates), per_family=True, precision="long")
+    for mode, tol in (("fp32", 1e-6), ("fp64s", 1e-12)):
+        got = w.logpdf_families(m, c, mode=mode)
+        err = np.max(np.abs(got - ref) / np.abs(ref))
+        print(mode, "max rel err vs long double", err)
+        assert err <= tol
+
+
+def test_config2_gradient(w, cfg2):
+    wl, om, oc, m, c = cfg2
+    ov, og = om.gradient(oc, orates(wl.rates), nthreads=8)
+    for mode, tol in (("fp64", 1e-10), ("fp32", 1e-5)):
+        v, g, _ = w.gradient(m, c, mode=mode)
+        err = np.max(np.abs(g - og)) / np.max(np.abs(og))
+        print(mode, "grad rel err", err)
+        assert err <= tol
+
+
+@pytest.mark.parametrize("cond", ["root", "nonextinct", "none"])
+def test_conditions(w, cond):
+    cls = {"root": w.RootCondition, "nonextinct": w.NonExtinctCondition, "none": w.NoCondition}[cond]
+    t = w.readnw(EXTREE)
+    for name, a, b in KAT_INSERTS:
+        w.insertnode(w.getlca(t, a, b), name=name)
+    rng = np.random.default_rng(11)
+    lam = list(rng.normal(-0.5, 0.4, 17))
+    mu = list(rng.normal(-0.7, 0.4, 17))
+    r = w.DLWGD(λ=lam, μ=mu, q=[0.3, 0.6], η=0.8)
+    m = w.WhaleModel(r, t, 0.05, condition=cls())
+    c = w.read_ale(os.path.join(GOLDEN, "example-1"), m)
+    om = OracleModel(EXTREE, KAT_INSERTS, 0.05, condition=cond)
+    oc = om.ccds(example1_texts())
+    orr = ODLWGD(lam, mu, [0.3, 0.6], 0.8)
+    ref = om.logpdf(oc, orr)
+    assert abs(w.logpdf(m, c) - ref) <= 1e-12 * abs(ref)
+    ov, og = om.gradient(oc, orr)
+    v, g, _ = w.gradient(m, c)
+    assert np.max(np.abs(g - og)) <= 1e-10 * np.max(np.abs(og))
+
+
+def test_constant_rates_gradient(w):
+    t = w.readnw(EXTREE)
+    for name, a, b in KAT_INSERTS:
+        w.insertnode(w.getlca(t, a, b), name=name)
+    r = w.ConstantDLWGD(λ=0.7, μ=0.4, q=[0.2, 0.1], η=0.8)
+    m = w.WhaleModel(r, t, 0.05)
+    c = w.read_ale(os.path.join(GOLDEN, "example-1"), m)
+    om = OracleModel(EXTREE, KAT_INSERTS, 0.05)
+    oc = om.ccds(example1_texts())
+    ov, og = om.gradient(oc, OConst(0.7, 0.4, [0.2, 0.1], 0.8))
+    v, g, _ = w.gradient(m, c)
+    assert abs(v - ov) <= 1e-12 * abs(ov)
+    assert np.max(np.abs(g - og)) <= 1e-10 * np.max(np.abs(og))
+
+
+def test_equal_rates_branch_semantics(w):
+    # at λ = μ the λΔ/(1+λΔ) branch of getα is differentiated, as ForwardDiff does
+    t = w.readnw(EXTREE)
+    for name, a, b in KAT_INSERTS:
+        w.insertnode(w.getlca(t, a, b), name=name)
+    n = 17
+    x = np.concatenate([np.ones(n), np.ones(n), [0.1, 0.2], [0.8]])
+    r = w.DLWGD(λ=list(x[:n]), μ=list(x[n:2 * n]), q=[0.1, 0.2], η=0.8)
+    m = w.WhaleModel(r, t, 0.05)
+    c = w.read_ale(os.path.join(GOLDEN, "example-1"), m)
+    om = OracleModel(EXTREE, KAT_INSERTS, 0.05)
+    oc = om.ccds(example1_texts())
+    ov, og = om.gradient(oc, ODLWGD(list(x[:n]), list(x[n:2 * n]), [0.1, 0.2], 0.8))
+    v, g, _ = w.gradient(m, c)
+    assert np.max(np.abs(g - og)) <= 1e-10 * np.max(np.abs(og))
+
+
+def test_mul_tree_parity(w):
+    t = w.readnw(MULTREE)
+    n = len(w.postwalk(t))
+    rng = np.random.default_rng(5)
+    lam, mu = list(rng.normal(-1, 0.3, n)), list(rng.normal(-1, 0.3, n))
+    m = w.WhaleModel(w.DLWGD(λ=lam, μ=mu, η=0.9), t, 0.05)
+    c = w.read_ale(os.path.join(GOLDEN, "example-1"), m)
+    om = OracleModel(MULTREE, [], 0.05)
+    oc = om.ccds(example1_texts())
+    ref = om.logpdf(oc, ODLWGD(lam, mu, [], 0.9), per_family=True)
+    for mode, tol in (("fp64", 1e-12), ("fp32", 1e-6), ("fp64s", 1e-12)):
+        got = w.logpdf_families(m, c, mode=mode)
+        assert np.max(np.abs(got - ref) / np.abs(ref)) <= tol
+    ov, og = om.gradient(oc, ODLWGD(lam, mu, [], 0.9))
+    v, g, _ = w.gradient(m, c)
+    assert np.max(np.abs(g - og)) <= 1e-10 * np.max(np.abs(og))
+
+
+def test_ell_download_matches_oracle(w):
+    t = w.readnw(EXTREE)
+    for name, a, b in KAT_INSERTS:
+        w.insertnode(w.getlca(t, a, b), name=name)
+    r = w.DLWGD(λ=[0.1] * 17, μ=[-0.2] * 17, q=[0.2, 0.1], η=0.9)
+    m = w.WhaleModel(r, t, 0.05)
+    c = w.read_ale(os.path.join(GOLDEN, "example-1"), m)
+    om = OracleModel(EXTREE, KAT_INSERTS, 0.05)
+    oc = om.ccds(example1_texts())
+    for mode, tol in (("fp64", 1e-13), ("fp64s", 1e-13), ("fp32", 1e-5)):
+        w.logpdf_(m, c, mode=mode)
+        for f in (0, 5, 11):
+            ref = om.ell(oc, f, ODLWGD([0.1] * 17, [-0.2] * 17, [0.2, 0.1], 0.9))
+            flat = np.concatenate([c[f].ell[i].T.ravel() for i in range(1, 20)])
+            assert flat.shape == ref.shape
+            scale = np.maximum(np.abs(ref), 1e-300)
+            big = np.abs(ref) > 1e-30 * np.max(np.abs(ref))
+            assert np.max(np.abs(flat - ref)[big] / scale[big]) <= tol
+
+
+def test_discretization_convergence(w):
+    t = open(os.path.join(GOLDEN, "example-5", "tree.nw")).readline().strip()
+    t1 = re.sub(r":[0-9.eE+-]+", ":1", t).rstrip(";") + ":1;"
+    rng = np.random.default_rng(7)
+    for _ in range(2):
+        tree = w.readnw(t1)
+        n = len(w.postwalk(tree))
+        rr = list(rng.standard_normal(n))
+        vals = []
+        for e in np.arange(-5, -0.99, 0.5):
+            m = w.WhaleModel(w.DLWGD(λ=rr, μ=rr, η=0.9), w.readnw(t1), 10.0 ** e)
+            c = w.read_ale(os.path.join(GOLDEN, "example-5", "OG0014587.ale"), m)
+            vals.append(w.logpdf_(m, c))
+            om = OracleModel(t1, [], 10.0 ** e)
+            ref = om.logpdf(om.ccds([open(os.path.join(GOLDEN, "example-5", "OG0014587.ale")).read()]),
+                            ODLWGD(rr, rr, [], 0.9))
+            assert abs(vals[-1] - ref) <= 1e-11 * abs(ref)
+        vals = np.array(vals)
+        assert np.all(np.abs(vals - vals[0]) < 0.1)

@@ -43,6 +43,8 @@ def lib():
         L.whale_ctx_stream.restype = vp
         L.whale_ctx_stream.argtypes = [vp]
         L.whale_ctx_set_timing.argtypes = [vp, C.c_int32]
+        L.whale_ctx_record.argtypes = [vp, C.c_int32]
+        L.whale_ctx_elapsed.argtypes = [vp, C.c_int32, C.c_int32, dblp]
         L.whale_last_timings.argtypes = [vp, dblp]
         L.whale_model_create.argtypes = [vp, C.c_int32, i32p, i32p, i32p, i32p, i32p, dblp, dblp, i32p, i32p,
                                          C.c_int32, i32p, C.c_int32, C.POINTER(vp)]

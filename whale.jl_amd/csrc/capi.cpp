@@ -65,6 +65,7 @@ struct whale_ctx {
   int n_cu = 256;
   bool timing = false;
   hipEvent_t ev[6] = {};
+  hipEvent_t user[16] = {};
   double last_ms[5] = {0, 0, 0, 0, 0};
 };
 
@@ -161,6 +162,7 @@ int whale_ctx_create(int device, whale_ctx** out) {
   HIPCHK(hipGetDeviceProperties(&prop, device));
   c->n_cu = prop.multiProcessorCount;
   for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+  for (auto& e : c->user) HIPCHK(hipEventCreate(&e));
   *out = c.release();
   return WHALE_OK;
 }
@@ -169,12 +171,30 @@ int whale_ctx_destroy(whale_ctx* c) {
   if (!c) return WHALE_OK;
   (void)hipSetDevice(c->device);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->user) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return WHALE_OK;
 }
 
 void* whale_ctx_stream(whale_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int whale_ctx_record(whale_ctx* c, int32_t slot) {
+  if (!c || slot < 0 || slot >= 16) return fail(WHALE_EINVAL, "bad context or event slot");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipEventRecord(c->user[slot], c->stream));
+  return WHALE_OK;
+}
+
+int whale_ctx_elapsed(whale_ctx* c, int32_t a, int32_t b, double* ms) {
+  if (!c || !ms || a < 0 || a >= 16 || b < 0 || b >= 16) return fail(WHALE_EINVAL, "bad context or event slot");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipEventSynchronize(c->user[b]));
+  float f = 0;
+  HIPCHK(hipEventElapsedTime(&f, c->user[a], c->user[b]));
+  *ms = f;
+  return WHALE_OK;
+}
 
 int whale_ctx_set_timing(whale_ctx* c, int32_t en) {
   if (!c) return fail(WHALE_EINVAL, "ctx is NULL");
