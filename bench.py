@@ -46,6 +46,47 @@ def algorithmic_bytes(st, s_bytes, x_fwd=0, x_bwd=0):
     return fwd, grad
 
 
+def copy_probe(nbytes=1 << 28):
+    """One device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE in a
+    rocprofv3 --pmc run give the counters' scale (KiB read per byte moved)."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {nbytes} bytes device-to-device")
+
+
+def sampler(w, _lib, synth, args):
+    """Sampler-style loop (config5): device-resident CCDs, parameter points
+    drawn as log-normal perturbations of the truth, evaluated back to back."""
+    wl = synth.workload("config5" if args.workload == "config4" else args.workload, n_families=args.families)
+    model = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt)
+    ccds = w.ccds_from_texts(wl.texts, model)
+    batch = w.device_batch(model, ccds)
+    n_fam = batch.n
+    points = synth.perturbed(wl.rates, 256)
+    models = [model(r) for r in points]
+    for m in models[:3]:
+        w.gradient(m, ccds, mode="fp32")
+    lat = []
+    t0 = time.perf_counter()
+    for m in models:
+        t = time.perf_counter()
+        w.gradient(m, ccds, mode="fp32")
+        lat.append(time.perf_counter() - t)
+    total = time.perf_counter() - t0
+    lat = np.array(lat) * 1e3
+    out = {"metric": "sampler loop: logL+grad per parameter point (FP32-scaled, batched)",
+           "workload": f"{wl.name}: {n_fam} families, {len(points)} points",
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "evals_per_s": len(points) / total, "families_per_s": n_fam * len(points) / total}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,6 +97,10 @@ def main():
     ap.add_argument("--no-per-family", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--families", type=int, default=None)
+    ap.add_argument("--sampler", action="store_true",
+                    help="config5 sampler loop: 256 perturbed parameter points, logL+grad each; p50/p99 latency")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one device-to-device copy of 256 MiB (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (families would be sharded across "
@@ -67,6 +112,8 @@ def main():
     from whale_jl_amd import _lib, synth
     from whale_jl_amd._lib import check, dblp, ptr
     L = _lib.lib()
+    if args.sampler:
+        return sampler(w, _lib, synth, args)
 
     t0 = time.time()
     wl = synth.workload(args.workload, n_families=args.families)
@@ -101,36 +148,40 @@ def main():
     v64s, f64s, g64s = step(_lib.FP64_SCALED)
     v64, f64, g64 = step(_lib.FP64_PLAIN)
     v32, f32, g32 = step(_lib.FP32_SCALED)
+    n_fallback = batch.fallbacks()
     fin = np.isfinite(f64s)
     rel_fam = float(np.max(np.abs(f32[fin] - f64s[fin]) / np.abs(f64s[fin])))
     rel_tot = abs(v32 - v64s) / abs(v64s)
     gerr = float(np.max(np.abs(g32 - g64s)) / np.max(np.abs(g64s)))
     n_underflow = int(np.sum(~np.isfinite(f64)))
     log(f"logL fp64s {v64s:.6f} fp64 {v64:.6f} fp32 {v32:.6f}; fp32 rel err family max {rel_fam:.2e} "
-        f"total {rel_tot:.2e} grad {gerr:.2e}; fp64-plain underflowed families {n_underflow}")
+        f"total {rel_tot:.2e} grad {gerr:.2e}; fp64-plain underflowed families {n_underflow}; "
+        f"fp32 families recomputed in fp64 {n_fallback}")
 
-    # ---- timed: batched FP32-scaled logL + gradient
+    # ---- timed: batched FP32-scaled logL + gradient (captured HIP graph replays)
     for _ in range(args.warmup):
         step(_lib.FP32_SCALED)
-    L.whale_ctx_set_timing(ctx, 1)
-    ms5 = np.zeros(5)
-    kern = np.zeros(5)
-    # HIP events on the library's stream bracket the timed steps
     t_wall = time.perf_counter()
-    check(L.whale_ctx_record(ctx, 0), "record")
+    check(L.whale_ctx_record(ctx, 0), "record")  # HIP events on the library's stream
     for _ in range(args.steps):
         step(_lib.FP32_SCALED)
-        L.whale_last_timings(ctx, ptr(ms5, dblp))
-        kern += ms5
     check(L.whale_ctx_record(ctx, 1), "record")
     ms_c = C.c_double()
     check(L.whale_ctx_elapsed(ctx, 0, 1, C.byref(ms_c)), "elapsed")
     wall = time.perf_counter() - t_wall
-    ms_total = ms_c.value
-    L.whale_ctx_set_timing(ctx, 0)
-    ms_step = ms_total / args.steps
-    kern /= args.steps  # [tables, forward, backward, tail, total]
+    ms_step = ms_c.value / args.steps
     value = n_fam / (ms_step / 1e3)
+    # per-kernel breakdown: the same steps launched eagerly with events between the kernels
+    L.whale_ctx_set_timing(ctx, 1)
+    ms5 = np.zeros(5)
+    kern = np.zeros(5)
+    nk = max(3, args.steps // 2)
+    for _ in range(nk):
+        step(_lib.FP32_SCALED)
+        L.whale_last_timings(ctx, ptr(ms5, dblp))
+        kern += ms5
+    L.whale_ctx_set_timing(ctx, 0)
+    kern /= nk  # [tables, forward, backward, tail, total]
     log(f"fp32-scaled batched: {ms_step:.3f} ms/step (wall {1e3 * wall / args.steps:.3f}), {value:.4g} families/s; "
         f"kernels ms: tables {kern[0]:.3f} fwd {kern[1]:.3f} bwd {kern[2]:.3f} tail {kern[3]:.3f}")
 
@@ -149,6 +200,9 @@ def main():
         per_fam = n_fam / pf_s
         log(f"fp64 plain per-family launches: {pf_s:.3f} s, {per_fam:.4g} families/s; "
             f"fp64 plain batched {fp64_batched_ms:.3f} ms")
+
+    if args.copy_probe:
+        copy_probe()
 
     # ---- roofline of the dominant kernel (algorithmic bytes / its duration)
     b_fwd, b_grad = algorithmic_bytes(st, 4, x_fwd=4 * st["clades"])
@@ -206,7 +260,8 @@ def main():
                      "algorithmic_bytes": traffic_bytes},
         "cpu_baseline": cpu,
         "accuracy": {"fp32_vs_fp64_family_max_rel": rel_fam, "fp32_vs_fp64_total_rel": rel_tot,
-                     "fp32_vs_fp64_grad_rel": gerr, "fp64_plain_underflowed_families": n_underflow},
+                     "fp32_vs_fp64_grad_rel": gerr, "fp64_plain_underflowed_families": n_underflow,
+                     "fp32_families_recomputed_in_fp64": n_fallback},
         "fp64_plain_per_family_families_per_s": per_fam,
         "speedup_vs_fp64_per_family": (value / per_fam) if per_fam else None,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,

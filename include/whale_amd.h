@@ -138,6 +138,10 @@ int whale_batch_destroy(whale_batch* b);
  * items, records, stages, row_steps, max_stage_items, ell_cells]. */
 int whale_batch_stats(const whale_batch* b, int64_t* out12);
 
+/* Number of families the last FP32-scaled call recomputed in FP64 because
+ * their FP32 values left the representable range.  No reference counterpart. */
+int whale_batch_fallbacks(const whale_batch* b, int32_t* n);
+
 /* logpdf of every family (unconditioned, src/core.jl:31-43) into
  * out_family[n_families] (may be NULL) and the conditioned batch value
  * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */

@@ -170,3 +170,27 @@ def test_discretization_convergence(w):
             assert abs(vals[-1] - ref) <= 1e-11 * abs(ref)
         vals = np.array(vals)
         assert np.all(np.abs(vals - vals[0]) < 0.1)
+
+
+def test_fp32_fallback_for_huge_paralog_expansion(w):
+    """64 in-paralogs on a long, finely sliced branch: the clade values grow far
+    beyond FP32's range after their exponents are fixed; such families are
+    recomputed in FP64 with per-clade exponents."""
+    from ale_util import ale_from_tree, balanced_paralogs
+    adj, names = balanced_paralogs(64, "A", ("B_1", "B_2"))
+    text = ale_from_tree(adj, names)
+    tree = "(A:1.0,B:1.0);"
+    om = OracleModel(tree, [], 0.005)
+    oc = om.ccds([text])
+    r = ODLWGD([np.log(0.8)] * 3, [np.log(0.3)] * 3, [], 0.9)
+    ref = om.logpdf(oc, r, per_family=True, precision="long")[0]
+    m = w.WhaleModel(w.DLWGD(λ=[np.log(0.8)] * 3, μ=[np.log(0.3)] * 3, η=0.9), w.readnw(tree), 0.005)
+    c = w.ccds_from_texts([text], m)
+    got = w.logpdf_families(m, c, mode="fp32")[0]
+    b = w.device_batch(m, c)
+    print("huge paralog family: ref", ref, "fp32", got, "recomputed in fp64:", b.fallbacks())
+    assert b.fallbacks() == 1
+    assert abs(got - ref) <= 1e-9 * abs(ref)
+    v, g, _ = w.gradient(m, c, mode="fp32")
+    v64, g64, _ = w.gradient(m, c, mode="fp64s")
+    assert np.max(np.abs(g - g64)) <= 1e-9 * np.max(np.abs(g64))

@@ -65,3 +65,26 @@ def test_config1_gradient_vs_dual(w, cfg1):
         print(mode, "value", v, ov, "grad rel err", err)
         assert abs(v - ov) <= (1e-12 if mode != "fp32" else 1e-6) * abs(ov)
         assert err <= tol
+
+
+def test_graph_replays_match_eager(w, cfg1):
+    """The batched evaluation is a captured HIP graph unless per-kernel timing is
+    on; replays interleaved with eager runs must agree with them every time."""
+    from whale_jl_amd import _lib
+    wl, om, oc, m, c = cfg1
+    ctx = w.context()
+    results = []
+    for rep in range(4):
+        for timing in (0, 1):
+            _lib.lib().whale_ctx_set_timing(ctx, timing)
+            for mode in ("fp64", "fp32"):
+                v, g, fam = w.gradient(m, c, mode=mode)
+                results.append((rep, timing, mode, v, g, fam))
+    _lib.lib().whale_ctx_set_timing(ctx, 0)
+    for mode in ("fp64", "fp32"):
+        rs = [r for r in results if r[2] == mode]
+        base = rs[0]
+        for r in rs[1:]:
+            assert np.array_equal(r[5], base[5])  # per-family logL: no atomics, bit-identical
+            assert abs(r[3] - base[3]) <= 1e-12 * abs(base[3])
+            assert np.max(np.abs(r[4] - base[4])) <= 1e-9 * np.max(np.abs(base[4]))

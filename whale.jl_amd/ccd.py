@@ -133,6 +133,12 @@ class DeviceBatch:
                 "row_steps", "max_stage_items", "ell_cells"]
         return dict(zip(keys, (int(x) for x in o)))
 
+    def fallbacks(self) -> int:
+        """Families the last FP32 call recomputed in FP64 (their FP32 values left the range)."""
+        n = C.c_int32()
+        check(_lib.lib().whale_batch_fallbacks(self.h, C.byref(n)), "whale_batch_fallbacks")
+        return n.value
+
     def ell(self, fam: int) -> np.ndarray:
         n = C.c_int64()
         check(_lib.lib().whale_download_ell(self.h, fam, None, 0, C.byref(n)), "whale_download_ell")

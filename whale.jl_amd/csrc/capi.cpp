@@ -101,13 +101,26 @@ struct whale_batch {
   PlanHost plan;
   DBuf fam, stage, seg, it_seg, it_gid, it_rec, it_inv, it_cf, it_cg, it_bar, rec_ops, rec_sid, inv_ops, inv_rec,
       rec_p, clade_size, split_p, lvl, order;
-  // workspace
-  DBuf ell, coef, E, ibar, fam_ll, acc, ticket;
+  // workspace.  ell/coef/ibar hold the main pass (FP32 or FP64 values) in
+  // their first 8 bytes per element and, for FP32 runs, the FP64 recomputation
+  // of families that left the FP32 range after the first align8(4 * n) bytes.
+  DBuf ell, coef, E, ibar, fam_ll, acc, ticket, flag, fb;
+  uint64_t n_rec = 0;
   uint32_t n_fam = 0, n_large = 0;
-  uint32_t cap_small = 0, cap_large = 0;
+  uint32_t cap_small = 0, cap_large = 0, rcap_small = 0, rcap_large = 0;
   int last_mode = -1;
   bool last_store_all = false;
+  uint32_t last_fallback = 0;  // families of the last FP32 call recomputed in FP64
   std::vector<uint32_t> order_host;
+  // captured evaluations, keyed by (mode, gradient, keep ℓ), and pinned result staging
+  hipGraphExec_t graph[3][2][2] = {};
+  double* pin_out = nullptr;
+  double* pin_fam = nullptr;
+  ~whale_batch() {
+    for (auto& a : graph) for (auto& b2 : a) for (auto& g : b2) if (g) (void)hipGraphExecDestroy(g);
+    if (pin_out) (void)hipHostFree(pin_out);
+    if (pin_fam) (void)hipHostFree(pin_fam);
+  }
   BatchDev dev() const {
     BatchDev B;
     B.fam = fam.as<FamDev>();
@@ -141,6 +154,17 @@ struct whale_batch {
     W.fam_ll = fam_ll.as<double>();
     W.acc = acc.as<double>();
     W.ticket = ticket.as<uint32_t>();
+    W.flag = flag.as<uint32_t>();
+    W.fb = fb.as<uint32_t>();
+    return W;
+  }
+  // the FP64 region used for families recomputed after an FP32 pass
+  WorkRaw work_fallback() const {
+    WorkRaw W = work();
+    auto off = [](uint64_t n) { return ((4 * n + 7) / 8) * 8; };
+    W.ell = (char*)ell.p + off(plan.ell_cells);
+    W.coef = (char*)coef.p + off(n_rec);
+    W.ibar = (char*)ibar.p + off(plan.ibar_cells);
     return W;
   }
 };
@@ -454,14 +478,17 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   b->n_fam = (uint32_t)n_fam;
   // size classes: families whose largest stage fits the small LDS cap run in a
   // launch with many workgroups per CU; the rest get their own launch.
-  const uint32_t small_cap = 512;
+  const uint32_t small_cap = 512, small_rcap = 1024;
   std::vector<uint32_t> large, small;
-  for (uint32_t f : P.cost_order) (P.fam[f].max_items > small_cap ? large : small).push_back(f);
+  for (uint32_t f : P.cost_order)
+    (P.fam[f].max_items > small_cap || P.fam[f].max_recs > small_rcap ? large : small).push_back(f);
   b->order_host = large;
   b->order_host.insert(b->order_host.end(), small.begin(), small.end());
   b->n_large = (uint32_t)large.size();
   b->cap_small = small_cap;
-  b->cap_large = P.max_items;
+  b->rcap_small = small_rcap;
+  b->cap_large = std::max(P.max_items, 1u);
+  b->rcap_large = std::max(P.max_recs, 2u);
   whale_ctx* c = m->ctx;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
@@ -484,14 +511,19 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   HIPCHK(upload(b->split_p, P.split_p, st));
   HIPCHK(upload(b->lvl, P.lvl, st));
   HIPCHK(upload(b->order, b->order_host, st));
-  HIPCHK(b->ell.alloc(sizeof(double) * std::max<uint64_t>(1, P.ell_cells)));
-  HIPCHK(b->coef.alloc(sizeof(double) * std::max<size_t>(1, P.rec_ops.size())));
+  b->n_rec = P.rec_ops.size();
+  HIPCHK(b->ell.alloc(16 + 12 * std::max<uint64_t>(1, P.ell_cells)));
+  HIPCHK(b->coef.alloc(16 + 12 * std::max<uint64_t>(1, b->n_rec)));
   HIPCHK(b->E.alloc(sizeof(int32_t) * std::max<int64_t>(1, nc)));
-  HIPCHK(b->ibar.alloc(sizeof(double) * std::max<uint64_t>(1, P.ibar_cells)));
+  HIPCHK(b->ibar.alloc(16 + 12 * std::max<uint64_t>(1, P.ibar_cells)));
+  HIPCHK(b->flag.alloc(sizeof(uint32_t) * n_fam));
+  HIPCHK(b->fb.alloc(sizeof(uint32_t) * (n_fam + 1)));
   HIPCHK(b->fam_ll.alloc(sizeof(double) * n_fam));
   AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   HIPCHK(b->acc.alloc(sizeof(double) * AL.n));
   HIPCHK(b->ticket.alloc(sizeof(uint32_t) * 8));
+  HIPCHK(hipHostMalloc((void**)&b->pin_out, sizeof(double) * (3 + 2 * m->h.n + m->h.n_wgd), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&b->pin_fam, sizeof(double) * n_fam, hipHostMallocDefault));
   // release host copies the device now owns (keep what ℓ download needs)
   std::vector<uint32_t>().swap(P.it_rec);
   std::vector<uint32_t>().swap(P.it_inv);
@@ -511,6 +543,12 @@ int whale_batch_destroy(whale_batch* b) {
   if (!b) return WHALE_OK;
   (void)hipSetDevice(b->device);  // the model may already be gone
   delete b;
+  return WHALE_OK;
+}
+
+int whale_batch_fallbacks(const whale_batch* b, int32_t* n) {
+  if (!b || !n) return fail(WHALE_EINVAL, "NULL argument");
+  *n = (int32_t)b->last_fallback;
   return WHALE_OK;
 }
 
@@ -534,12 +572,96 @@ int whale_batch_stats(const whale_batch* b, int64_t* o) {
 
 namespace {
 
-int grid_for(const whale_batch* b, int mode, uint32_t cap, uint32_t nfam) {
+int grid_for(const whale_batch* b, int mode, uint32_t cap, uint32_t rcap, uint32_t nfam) {
   // workgroups resident per CU: LDS-limited, at most 16 one-wave workgroups
-  size_t lds = std::max<size_t>(1, lds_bytes(mode, cap));
+  size_t lds = std::max<size_t>(1, lds_bytes(mode, cap, rcap));
   int per_cu = (int)std::min<size_t>(16, std::max<size_t>(1, (160 * 1024) / lds));
   int g = b->model->ctx->n_cu * per_cu;
   return (int)std::min<uint32_t>((uint32_t)g, std::max<uint32_t>(1u, nfam));
+}
+
+// All device work of one evaluation on the context's stream (capturable: no
+// memset nodes, no host synchronisation).  Timing events only when `timed`.
+int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all, bool timed) {
+  whale_model* m = b->model;
+  whale_ctx* c = m->ctx;
+  hipStream_t st = c->stream;
+  const BatchDev B = b->dev();
+  const ModelDev M = m->dev();
+  const WorkRaw W = b->work();
+  const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+  if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
+  HIPCHK(launch_fill_u32(st, W.ticket, 0u, 8));
+  if (grad) HIPCHK(launch_fill_f64(st, W.acc, 0.0, AL.n));
+  const uint32_t nl = b->n_large, nf = b->n_fam;
+  const bool fp32 = mode == WHALE_FP32_SCALED;
+  if (fp32) {
+    HIPCHK(launch_fill_u32(st, W.flag, 0u, (int)nf));
+    HIPCHK(launch_fill_u32(st, W.fb, 0u, 1));
+  }
+  // families the FP32 pass could not represent, recomputed in FP64 with per-clade exponents
+  BatchDev Bfb = B;
+  Bfb.order = W.fb + 1;
+  WorkRaw Wfb = b->work_fallback();
+  Wfb.ticket = W.ticket + 4;
+  const int fb_grid = std::max(1, c->n_cu);
+  if (launch == WHALE_LAUNCH_BATCHED) {
+    if (nl)
+      HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_large, b->rcap_large, nl), b->cap_large,
+                            b->rcap_large, st, B, M, W, -1, 0, nl));
+    if (nf > nl) {
+      WorkRaw W2 = W;
+      W2.ticket = W.ticket + 1;
+      HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_small, b->rcap_small, nf - nl), b->cap_small,
+                            b->rcap_small, st, B, M, W2, -1, nl, nf));
+    }
+  } else {
+    for (uint32_t k = 0; k < nf; ++k) {
+      uint32_t f = b->order_host[k];
+      const FamDev& fd = b->plan.fam[f];
+      HIPCHK(launch_forward(mode, store_all, 1, fd.max_items, std::max(fd.max_recs, 2u), st, B, M, W, (int32_t)f, 0, 0));
+    }
+  }
+  if (fp32)
+    HIPCHK(launch_forward(WHALE_FP64_SCALED, store_all, fb_grid, b->cap_large, b->rcap_large, st, Bfb, M, Wfb, -1, 0,
+                          0, W.fb));
+  if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
+  if (grad) {
+    if (launch == WHALE_LAUNCH_BATCHED) {
+      WorkRaw W3 = W;
+      W3.ticket = W.ticket + 2;
+      if (nl)
+        HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_large, b->rcap_large, nl), b->cap_large, b->rcap_large,
+                               st, B, M, W3, -1, 0, nl));
+      if (nf > nl) {
+        W3.ticket = W.ticket + 3;
+        HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_small, b->rcap_small, nf - nl), b->cap_small,
+                               b->rcap_small, st, B, M, W3, -1, nl, nf));
+      }
+    } else {
+      for (uint32_t k = 0; k < nf; ++k) {
+        uint32_t f = b->order_host[k];
+        const FamDev& fd = b->plan.fam[f];
+        HIPCHK(launch_backward(mode, 1, fd.max_items, std::max(fd.max_recs, 2u), st, B, M, W, (int32_t)f, 0, 0));
+      }
+    }
+    if (fp32) {
+      WorkRaw Wfb2 = Wfb;
+      Wfb2.ticket = W.ticket + 5;
+      HIPCHK(launch_backward(WHALE_FP64_SCALED, fb_grid, b->cap_large, b->rcap_large, st, Bfb, M, Wfb2, -1, 0, 0,
+                             W.fb));
+    }
+  }
+  if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
+  double* out = m->out.as<double>();
+  HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
+                     m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out));
+  if (timed) HIPCHK(hipEventRecord(c->ev[4], st));
+  const int n = m->h.n, nw = m->h.n_wgd;
+  HIPCHK(hipMemcpyAsync(b->pin_out, out, sizeof(double) * (grad ? 2 + 2 * n + nw : 1), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(b->pin_fam, W.fam_ll, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&b->pin_out[2 + 2 * n + nw], W.fb, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  return WHALE_OK;
 }
 
 int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_family, double* out_total,
@@ -555,55 +677,25 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
   const bool store_all = grad || keep;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
-  const BatchDev B = b->dev();
-  const ModelDev M = m->dev();
-  const WorkRaw W = b->work();
-  const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
-  HIPCHK(launch_fill_u32(st, W.ticket, 0u, 8));
-  if (grad) HIPCHK(launch_fill_f64(st, W.acc, 0.0, AL.n));
-  const uint32_t nl = b->n_large, nf = b->n_fam;
-  if (launch == WHALE_LAUNCH_BATCHED) {
-    if (nl) HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_large, nl), b->cap_large, st, B, M, W, -1, 0, nl));
-    if (nf > nl) {
-      WorkRaw W2 = W;
-      W2.ticket = W.ticket + 1;
-      HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_small, nf - nl), b->cap_small, st, B, M, W2, -1, nl, nf));
+  if (launch == WHALE_LAUNCH_BATCHED && !c->timing) {
+    // replay the captured evaluation (captured on first use)
+    hipGraphExec_t& ge = b->graph[mode][grad ? 1 : 0][keep ? 1 : 0];
+    if (!ge) {
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+      int rc = enqueue(b, mode, launch, grad, store_all, false);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc != WHALE_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+      HIPCHK(e);
+      e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHK(e);
     }
+    HIPCHK(hipGraphLaunch(ge, st));
   } else {
-    for (uint32_t k = 0; k < nf; ++k) {
-      uint32_t f = b->order_host[k];
-      uint32_t cap = b->plan.fam[f].max_items;
-      HIPCHK(launch_forward(mode, store_all, 1, cap, st, B, M, W, (int32_t)f, 0, 0));
-    }
+    int rc = enqueue(b, mode, launch, grad, store_all, c->timing);
+    if (rc != WHALE_OK) return rc;
   }
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[2], st));
-  if (grad) {
-    if (launch == WHALE_LAUNCH_BATCHED) {
-      WorkRaw W3 = W;
-      W3.ticket = W.ticket + 2;
-      if (nl) HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_large, nl), b->cap_large, st, B, M, W3, -1, 0, nl));
-      if (nf > nl) {
-        W3.ticket = W.ticket + 3;
-        HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_small, nf - nl), b->cap_small, st, B, M, W3, -1, nl, nf));
-      }
-    } else {
-      for (uint32_t k = 0; k < nf; ++k) {
-        uint32_t f = b->order_host[k];
-        uint32_t cap = b->plan.fam[f].max_items;
-        HIPCHK(launch_backward(mode, 1, cap, st, B, M, W, (int32_t)f, 0, 0));
-      }
-    }
-  }
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[3], st));
-  double* out = m->out.as<double>();
-  HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
-                     m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[4], st));
-  const int n = m->h.n, nw = m->h.n_wgd;
-  std::vector<double> o(2 + 2 * n + nw);
-  HIPCHK(hipMemcpyAsync(o.data(), out, sizeof(double) * (grad ? o.size() : 1), hipMemcpyDeviceToHost, st));
-  if (out_family) HIPCHK(hipMemcpyAsync(out_family, W.fam_ll, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (c->timing) {
     float a, bb, cc, dd;
@@ -616,15 +708,19 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
     c->last_ms[3] = cc;
     c->last_ms[4] = dd;
   }
+  const int n = m->h.n, nw = m->h.n_wgd;
+  const double* o = b->pin_out;
   *out_total = o[0];
+  if (out_family) std::copy(b->pin_fam, b->pin_fam + b->n_fam, out_family);
   if (grad) {
-    if (d_lambda) std::copy(o.begin() + 1, o.begin() + 1 + n, d_lambda);
-    if (d_mu) std::copy(o.begin() + 1 + n, o.begin() + 1 + 2 * n, d_mu);
-    if (d_q) std::copy(o.begin() + 1 + 2 * n, o.begin() + 1 + 2 * n + nw, d_q);
+    if (d_lambda) std::copy(o + 1, o + 1 + n, d_lambda);
+    if (d_mu) std::copy(o + 1 + n, o + 1 + 2 * n, d_mu);
+    if (d_q) std::copy(o + 1 + 2 * n, o + 1 + 2 * n + nw, d_q);
     if (d_eta) *d_eta = o[1 + 2 * n + nw];
   }
   b->last_mode = mode;
   b->last_store_all = store_all;
+  b->last_fallback = mode == WHALE_FP32_SCALED ? *reinterpret_cast<const uint32_t*>(&o[2 + 2 * n + nw]) : 0;
   return WHALE_OK;
 }
 
@@ -655,13 +751,22 @@ int whale_download_ell(whale_batch* b, int32_t f, double* out, int64_t cap, int6
     if (segof[e] >= 0) total += (int64_t)m->h.nodes[e].R * P.seg[segof[e]].nc;
   *n_out = total;
   if (!out || cap < total) return WHALE_OK;
-  const int mode = b->last_mode;
+  int mode = b->last_mode;
+  HIPCHK(hipSetDevice(m->ctx->device));
+  const char* base = (const char*)b->ell.p;
+  if (mode == WHALE_FP32_SCALED) {
+    uint32_t flagged = 0;
+    HIPCHK(hipMemcpy(&flagged, b->flag.as<uint32_t>() + f, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (flagged) {
+      mode = WHALE_FP64_SCALED;
+      base = (const char*)b->work_fallback().ell;
+    }
+  }
   const size_t rs = mode == WHALE_FP32_SCALED ? 4 : 8;
   uint64_t cells = (f + 1 < (int32_t)b->n_fam ? P.fam[f + 1].ell0 : P.ell_cells) - F.ell0;
   std::vector<unsigned char> raw(cells * rs);
   std::vector<int32_t> E(F.n_clades, 0);
-  HIPCHK(hipSetDevice(m->ctx->device));
-  HIPCHK(hipMemcpy(raw.data(), (char*)b->ell.p + F.ell0 * rs, raw.size(), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(raw.data(), base + F.ell0 * rs, raw.size(), hipMemcpyDeviceToHost));
   if (mode != WHALE_FP64_PLAIN)
     HIPCHK(hipMemcpy(E.data(), b->E.as<int32_t>() + F.clade0, sizeof(int32_t) * F.n_clades, hipMemcpyDeviceToHost));
   // compatible clades of each segment, in increasing clade id (the same order as the reference's compat)

@@ -52,6 +52,8 @@ struct WorkRaw {
   double* fam_ll;
   double* acc;
   uint32_t* ticket;
+  uint32_t* flag;     // per family: 1 if the FP32 form left its range (recomputed in FP64)
+  uint32_t* fb;       // fb[0] = number of such families, fb[1..] = their indices
 };
 
 // gradient accumulator layout (doubles)
@@ -71,11 +73,15 @@ struct AccLayout {
 // forward/backward launch: `mode` is WHALE_FP64_PLAIN / WHALE_FP32_SCALED /
 // WHALE_FP64_SCALED; fam_single >= 0 runs that one family on a 1-block grid,
 // otherwise blocks take families order[fam_begin .. fam_end) from ticket.
-hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, hipStream_t st, const BatchDev& B,
-                          const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
-                          uint32_t fam_end);
-hipError_t launch_backward(int mode, int grid, uint32_t cap, hipStream_t st, const BatchDev& B, const ModelDev& M,
-                           const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end);
+// `cap` is the LDS capacity in items, `rcap` in records (of any non-root stage).
+// fam_count, if not null, replaces fam_end with a count read on the device
+// (the FP64 fallback pass over families listed by the FP32 pass).
+hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, uint32_t rcap, hipStream_t st,
+                          const BatchDev& B, const ModelDev& M, const WorkRaw& W, int32_t fam_single,
+                          uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count = nullptr);
+hipError_t launch_backward(int mode, int grid, uint32_t cap, uint32_t rcap, hipStream_t st, const BatchDev& B,
+                           const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
+                           uint32_t fam_end, const uint32_t* fam_count = nullptr);
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
                          double* dnode, const int32_t* hlevels, const int32_t* hnodes, int n_levels);
 hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,
@@ -83,6 +89,6 @@ hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, 
                        const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int n);
 hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n);
-size_t lds_bytes(int mode, uint32_t cap);
+size_t lds_bytes(int mode, uint32_t cap, uint32_t rcap);
 
 }  // namespace whale

@@ -31,6 +31,13 @@ namespace whale {
 
 constexpr int kUndef = -0x40000000;  // exponent not yet fixed
 constexpr int kWG = 64;              // one wave per family
+// A clade's exponent is fixed at its first non-zero value, which can be far
+// smaller than its later values (e.g. a large in-paralog clade grows by ~2^120
+// along one branch as more slices become available for its duplications).
+// New non-leaf clades therefore start at 2^-kBirth, leaving 2^(128+kBirth) of
+// growth and 2^-(126-kBirth) of decay before FP32 runs out of range; a family
+// that still leaves the range is recomputed in FP64 (see k_forward's flag).
+constexpr int kBirth = 40;
 
 template <typename Real>
 struct WorkDev {
@@ -41,9 +48,11 @@ struct WorkDev {
   double* fam_ll;
   double* acc;        // gradient accumulators
   uint32_t* ticket;
+  uint32_t* flag;
+  uint32_t* fb;
   explicit WorkDev(const WorkRaw& w)
       : ell((Real*)w.ell), coef((Real*)w.coef), E(w.E), ibar((Real*)w.ibar), fam_ll(w.fam_ll), acc(w.acc),
-        ticket(w.ticket) {}
+        ticket(w.ticket), flag(w.flag), fb(w.fb) {}
 };
 
 __device__ __forceinline__ double eps_top(const ModelDev& M, int node) {
@@ -208,17 +217,66 @@ __device__ __forceinline__ Real scaled_coef(const BatchDev& B, const FamDev& F, 
   return ldexp_r((Real)B.split_p[F.split0 + B.rec_sid[rec]], e1 + e2 - Eg);
 }
 
+// Every workgroup is a single wavefront (kWG == 64).  A wave issues its LDS
+// instructions in order and the LDS executes them in order, so between the
+// rows of a stage it is enough to stop the compiler from moving LDS accesses
+// across the row boundary; no s_barrier and no wait on outstanding global
+// stores is needed.  Stage boundaries (where global ℓ written earlier is read
+// back) keep __syncthreads().
+static_assert(kWG == 64, "row synchronisation assumes one wavefront per workgroup");
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// row-invariant data of one item, cached in registers for a stage's rows
+// ---------------------------------------------------------------------------
+constexpr int kCache = 4;  // items per lane held in registers (stages of up to 256 items)
+
+struct ItemRow {
+  uint32_t cell;    // ℓ cell of row 0, relative to the family
+  uint32_t stride;  // |C_e| of the item's segment (row pitch)
+  uint32_t tab;     // first table row of the segment's node
+  uint32_t r0, r1;  // records
+  uint32_t q0, q1;  // inverse records
+  uint32_t R;       // rows of the segment
+  uint32_t gid;     // family-local clade id
+  uint32_t seg;     // absolute segment (reduction key)
+};
+
+__device__ __forceinline__ ItemRow load_item(const BatchDev& B, const ModelDev& M, uint32_t ia, uint32_t i) {
+  ItemRow it;
+  it.seg = B.it_seg[ia];
+  const SegDev sg = B.seg[it.seg];
+  it.cell = sg.ell + (i - sg.item0);
+  it.stride = sg.nc;
+  it.tab = M.node[sg.node].tab;
+  it.r0 = B.it_rec[ia];
+  it.r1 = B.it_rec[ia + 1];
+  it.q0 = B.it_inv[ia];
+  it.q1 = B.it_inv[ia + 1];
+  it.R = sg.R;
+  it.gid = B.it_gid[ia];
+  return it;
+}
+
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
 template <typename Real, bool Scaled, bool StoreAll>
 __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, uint32_t cap,
-                                                  int32_t fam_single, uint32_t fam_begin, uint32_t fam_end) {
+                                                  uint32_t rcap, int32_t fam_single, uint32_t fam_begin,
+                                                  uint32_t fam_end, const uint32_t* fam_count) {
+  if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   Real* bA = reinterpret_cast<Real*>(smem);
   Real* bB = bA + cap;
   Real* bF = bB + cap;
   Real* bG = bF + cap;
+  Real* sCoef = bG + cap;                                    // rcap coefficients of the stage's records
+  uint32_t* sOps = reinterpret_cast<uint32_t*>(sCoef + rcap);  // rcap operand pairs
   __shared__ uint32_t s_fam;
   __shared__ int s_dirty;
   const int lane = threadIdx.x;
@@ -241,6 +299,18 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
     const FamDev F = B.fam[f];
     Real* ell = W.ell + F.ell0;
     int32_t* E = W.E;
+    // scaled forms: coefficients of the stage's records, into LDS and (for the reverse sweep) global
+    auto refresh_coefs = [&](const StageDev& S, uint32_t rb) {
+      for (uint32_t i = lane; i < S.n_items; i += kWG) {
+        const uint32_t ia = S.item0 + i;
+        const int Eg = E[F.clade0 + B.it_gid[ia]];
+        for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) {
+          const Real c = scaled_coef<Real>(B, F, E, S.item0, k, Eg);
+          sCoef[k - rb] = c;
+          W.coef[k] = c;
+        }
+      }
+    };
     if (Scaled) {
       for (uint32_t c = lane; c < F.n_clades; c += kWG) E[F.clade0 + c] = kUndef;
       __syncthreads();
@@ -249,10 +319,16 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
     for (uint32_t s = 0; s < F.n_stages; ++s) {
       const StageDev S = B.stage[F.stage0 + s];
       const uint32_t n = S.n_items;
+      const uint32_t rb = B.it_rec[S.item0];             // the stage's records are [rb, rb + nrec)
+      const uint32_t nrec = B.it_rec[S.item0 + n] - rb;  // <= rcap (size class)
       for (uint32_t i = lane; i < n; i += kWG) {
         int32_t cf = B.it_cf[S.item0 + i], cg = B.it_cg[S.item0 + i];
         bF[i] = cf >= 0 ? ell[cf] : Real(0);
         bG[i] = cg >= 0 ? ell[cg] : Real(0);
+      }
+      for (uint32_t k = lane; k < nrec; k += kWG) {
+        sOps[k] = B.rec_ops[rb + k];
+        if (!Scaled) sCoef[k] = coefs[rb + k];
       }
       __syncthreads();
       // row 1
@@ -284,18 +360,18 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
             const Real qq = (Real)M.par[3 * M.n + nd.wgd];
             const Real ef = (Real)eps_top(M, nd.child0);
             Real r = 0;
-            for (uint32_t k = r0; k < r1; ++k) {
-              uint32_t ops = B.rec_ops[k];
-              r += coefs[k] * bF[ops & 0xffff] * bF[ops >> 16];
+            for (uint32_t k = r0 - rb; k < r1 - rb; ++k) {
+              uint32_t ops = sOps[k];
+              r += sCoef[k] * bF[ops & 0xffff] * bF[ops >> 16];
             }
             v = r * qq + ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]);
           } else {
             const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
             Real sp = 0;
-            for (uint32_t k = r0; k < r1; ++k) {
-              uint32_t ops = B.rec_ops[k];
+            for (uint32_t k = r0 - rb; k < r1 - rb; ++k) {
+              uint32_t ops = sOps[k];
               uint32_t a = ops & 0xffff, b = ops >> 16;
-              sp += coefs[k] * (bF[a] * bG[b] + bG[a] * bF[b]);
+              sp += sCoef[k] * (bF[a] * bG[b] + bG[a] * bF[b]);
             }
             v = sp + (bF[i] * eg + bG[i] * ef);
           }
@@ -322,8 +398,8 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
           if (Eg == kUndef) {
             if (xs.k != kUndef) {
               int ex;
-              v = frexp_r(xs.s, &ex);
-              E[F.clade0 + gid] = xs.k + ex;
+              v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
+              E[F.clade0 + gid] = xs.k + ex + kBirth;
             }
           } else {
             if (wgd) base = ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]);
@@ -337,39 +413,37 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
       }
       __syncthreads();
       if (Scaled) {
-        for (uint32_t i = lane; i < n; i += kWG) {
-          const uint32_t ia = S.item0 + i;
-          int Eg = E[F.clade0 + B.it_gid[ia]];
-          for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) W.coef[k] = scaled_coef<Real>(B, F, E, S.item0, k, Eg);
-        }
+        refresh_coefs(S, rb);
         __syncthreads();
       }
       // rows 2..R
       Real* cur = bA;
       Real* nxt = bB;
+      ItemRow cache[kCache];
+#pragma unroll
+      for (int c = 0; c < kCache; ++c) {
+        const uint32_t i = lane + c * kWG;
+        if (i < n) cache[c] = load_item(B, M, S.item0 + i, i);
+      }
       for (uint32_t r = 1; r < S.r_max; ++r) {
-        if (Scaled && lane == 0) s_dirty = 0;
-        for (uint32_t i = lane; i < n; i += kWG) {
-          const uint32_t ia = S.item0 + i;
-          const SegDev sg = B.seg[B.it_seg[ia]];
+        if (Scaled && lane == 0) s_dirty = 0;  // one wave per workgroup: ordered after last row's read
+        auto update = [&](const ItemRow& it, uint32_t i) {
           Real v = cur[i];
-          if (r < sg.R) {
-            const NodeDev& nd = M.node[sg.node];
-            const Real ph = (Real)M.phi[nd.tab + r], ps = (Real)M.psi[nd.tab + r];
-            const uint32_t r0 = B.it_rec[ia], r1 = B.it_rec[ia + 1];
+          if (r < it.R) {
+            const Real ph = (Real)M.phi[it.tab + r], ps = (Real)M.psi[it.tab + r];
             bool pending = false;
-            if (Scaled && v == Real(0) && r1 > r0) pending = E[F.clade0 + B.it_gid[ia]] == kUndef;
+            if (Scaled && v == Real(0) && it.r1 > it.r0) pending = E[F.clade0 + it.gid] == kUndef;
             if (!pending) {
               Real acc = 0;
-              for (uint32_t k = r0; k < r1; ++k) {
-                uint32_t ops = B.rec_ops[k];
-                acc += coefs[k] * cur[ops & 0xffff] * cur[ops >> 16];
+              for (uint32_t k = it.r0 - rb; k < it.r1 - rb; ++k) {
+                uint32_t ops = sOps[k];
+                acc += sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
               }
               v = ph * v + ps * acc;
             } else {
               XSum<Real> xs;
-              for (uint32_t k = r0; k < r1; ++k) {
-                uint32_t ops = B.rec_ops[k];
+              for (uint32_t k = it.r0; k < it.r1; ++k) {
+                uint32_t ops = sOps[k - rb];
                 uint32_t a = ops & 0xffff, b = ops >> 16;
                 Real va = cur[a], vb = cur[b];
                 if (va == Real(0) || vb == Real(0)) continue;
@@ -380,26 +454,31 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
               }
               if (xs.k != kUndef) {
                 int ex;
-                v = frexp_r(ps * xs.s, &ex);
-                E[F.clade0 + B.it_gid[ia]] = xs.k + ex;
+                v = ldexp_r(frexp_r(ps * xs.s, &ex), -kBirth);
+                E[F.clade0 + it.gid] = xs.k + ex + kBirth;
                 s_dirty = 1;
               }
             }
-            if (StoreAll || r == sg.R - 1) ell[sg.ell + (size_t)r * sg.nc + (i - sg.item0)] = v;
+            if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
           }
           nxt[i] = v;
+        };
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n) update(cache[c], i);
         }
-        __syncthreads();
+        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) update(load_item(B, M, S.item0 + i, i), i);
+        wave_sync();
         if (Scaled && s_dirty) {
-          for (uint32_t i = lane; i < n; i += kWG) {
-            const uint32_t ia = S.item0 + i;
-            int Eg = E[F.clade0 + B.it_gid[ia]];
-            for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) W.coef[k] = scaled_coef<Real>(B, F, E, S.item0, k, Eg);
-          }
+          // an exponent was fixed in this row (E is global: full barrier)
+          __syncthreads();
+          refresh_coefs(S, rb);
           __syncthreads();
         }
         Real* t = cur; cur = nxt; nxt = t;
       }
+      __syncthreads();  // this stage's ℓ is read by later stages
     }
     // ---------------- root: all clades, by size level
     {
@@ -452,8 +531,8 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
               v = 0;
               if (xs.k != kUndef) {
                 int ex;
-                v = frexp_r(xs.s, &ex);
-                E[F.clade0 + i] = xs.k + ex;
+                v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
+                E[F.clade0 + i] = xs.k + ex + kBirth;
               }
             } else {
               Real c = bF[i] * eg + bG[i] * ef;
@@ -473,6 +552,11 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
           if (Scaled) ll += (double)E[F.clade0 + n - 1] * 0.69314718055994530942;
         } else {
           ll = -INFINITY;
+          if (Scaled && sizeof(Real) == sizeof(float)) {
+            // overflow, or everything underflowed: redo this family in FP64 (a true zero stays -Inf there)
+            W.flag[f] = 1u;
+            W.fb[1 + atomicAdd(W.fb, 1u)] = f;
+          }
         }
         W.fam_ll[f] = ll;
       }
@@ -494,12 +578,16 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
 // ---------------------------------------------------------------------------
 template <typename Real, bool Scaled>
 __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, uint32_t cap,
-                                                   int32_t fam_single, uint32_t fam_begin, uint32_t fam_end) {
+                                                   uint32_t rcap, int32_t fam_single, uint32_t fam_begin,
+                                                   uint32_t fam_end, const uint32_t* fam_count) {
+  if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   Real* bA = reinterpret_cast<Real*>(smem);  // d logL / d ℓ, current row
   Real* bB = bA + cap;                        // next row
   Real* bV = bB + cap;                        // ℓ of the row below / child0 tops
   Real* bG = bV + cap;                        // child1 tops
+  Real* sCoef = bG + cap;                                      // the stage's record coefficients
+  uint32_t* sOps = reinterpret_cast<uint32_t*>(sCoef + rcap);  // and operand pairs
   __shared__ uint32_t s_fam;
   const int lane = threadIdx.x;
   const Real* coefs = Scaled ? W.coef : reinterpret_cast<const Real*>(B.rec_p);
@@ -520,6 +608,7 @@ __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDe
       f = B.order[k];
     }
     first = false;
+    if (Scaled && sizeof(Real) == sizeof(float) && W.flag[f]) continue;  // done by the FP64 pass
     const FamDev F = B.fam[f];
     const Real* ell = W.ell + F.ell0;
     Real* ibar = W.ibar + F.ibar0;
@@ -605,51 +694,75 @@ __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDe
     for (int s = (int)F.n_stages - 1; s >= 0; --s) {
       const StageDev S = B.stage[F.stage0 + s];
       const uint32_t n = S.n_items;
+      const uint32_t rb = B.it_rec[S.item0];
+      const uint32_t nrec = B.it_rec[S.item0 + n] - rb;  // <= rcap (size class)
       for (uint32_t i = lane; i < n; i += kWG) bA[i] = ibar[B.it_bar[S.item0 + i]];
+      for (uint32_t k = lane; k < nrec; k += kWG) {
+        sOps[k] = B.rec_ops[rb + k];
+        sCoef[k] = coefs[rb + k];
+      }
       __syncthreads();
+      ItemRow cache[kCache];
+#pragma unroll
+      for (int c = 0; c < kCache; ++c) {
+        const uint32_t i = lane + c * kWG;
+        if (i < n) cache[c] = load_item(B, M, S.item0 + i, i);
+      }
       for (int r = (int)S.r_max - 1; r >= 1; --r) {
-        for (uint32_t i = lane; i < n; i += kWG) {
-          const uint32_t ia = S.item0 + i;
-          const SegDev sg = B.seg[B.it_seg[ia]];
-          if (r < (int)sg.R) bV[i] = ell[sg.ell + (size_t)(r - 1) * sg.nc + (i - sg.item0)];
+        // ℓ of the row below, for the items whose segment reaches row r
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n && r < (int)cache[c].R) bV[i] = ell[cache[c].cell + (size_t)(r - 1) * cache[c].stride];
         }
-        __syncthreads();
-        for (uint32_t base = 0; base < n; base += kWG) {
-          const uint32_t i = base + lane;
+        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) {
+          const ItemRow it = load_item(B, M, S.item0 + i, i);
+          if (r < (int)it.R) bV[i] = ell[it.cell + (size_t)(r - 1) * it.stride];
+        }
+        wave_sync();
+        // all lanes call this together (segmented reductions)
+        auto update = [&](const ItemRow& it, uint32_t i, bool valid) {
           Real cphi = 0, cpsi = 0;
           uint32_t key = kNone;
           double* addr = acc;
-          if (i < n) {
-            const uint32_t ia = S.item0 + i;
-            const uint32_t si = B.it_seg[ia];
-            const SegDev sg = B.seg[si];
+          if (valid) {
             Real g = bA[i];
-            if (r < (int)sg.R) {
-              const NodeDev& nd = M.node[sg.node];
-              const Real ph = (Real)M.phi[nd.tab + r], ps = (Real)M.psi[nd.tab + r];
+            if (r < (int)it.R) {
+              const Real ph = (Real)M.phi[it.tab + r], ps = (Real)M.psi[it.tab + r];
               Real sd = 0;
-              for (uint32_t k = B.it_inv[ia]; k < B.it_inv[ia + 1]; ++k) {
+              for (uint32_t k = it.q0; k < it.q1; ++k) {
                 uint32_t ops = B.inv_ops[k];
-                sd += coefs[B.inv_rec[k]] * bA[ops & 0xffff] * bV[ops >> 16];
+                sd += sCoef[B.inv_rec[k] - rb] * bA[ops & 0xffff] * bV[ops >> 16];
               }
               Real fo = 0;
-              for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) {
-                uint32_t ops = B.rec_ops[k];
-                fo += coefs[k] * bV[ops & 0xffff] * bV[ops >> 16];
+              for (uint32_t k = it.r0 - rb; k < it.r1 - rb; ++k) {
+                uint32_t ops = sOps[k];
+                fo += sCoef[k] * bV[ops & 0xffff] * bV[ops >> 16];
               }
               bB[i] = ph * g + ps * sd;
               cphi = g * bV[i];
               cpsi = g * fo;
-              key = si;
-              addr = acc + nd.tab + r;
+              key = it.seg;
+              addr = acc + it.tab + r;
             } else {
               bB[i] = g;
             }
           }
           seg_add(cphi, key, addr);
           seg_add(cpsi, key, addr + AL.psi);
+        };
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          if ((uint32_t)c * kWG < n) {
+            const uint32_t i = lane + c * kWG;
+            update(cache[c], i, i < n);
+          }
         }
-        __syncthreads();
+        for (uint32_t base = kCache * kWG; base < n; base += kWG) {
+          const uint32_t i = base + lane;
+          update(i < n ? load_item(B, M, S.item0 + i, i) : cache[0], i, i < n);
+        }
+        wave_sync();
         Real* t = bA; bA = bB; bB = t;
       }
       // row 1: adjoints of the child tops, ε and q
@@ -678,7 +791,7 @@ __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDe
             Real sf = 0, sgg = 0;
             for (uint32_t k = B.it_inv[ia]; k < B.it_inv[ia + 1]; ++k) {
               uint32_t ops = B.inv_ops[k];
-              Real c = coefs[B.inv_rec[k]] * bA[ops & 0xffff];
+              Real c = sCoef[B.inv_rec[k] - rb] * bA[ops & 0xffff];
               uint32_t sib = ops >> 16;
               sf += c * bG[sib];
               sgg += c * bV[sib];
@@ -696,13 +809,13 @@ __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDe
             Real sf = 0;
             for (uint32_t k = B.it_inv[ia]; k < B.it_inv[ia + 1]; ++k) {
               uint32_t ops = B.inv_ops[k];
-              sf += coefs[B.inv_rec[k]] * bA[ops & 0xffff] * bV[ops >> 16];
+              sf += sCoef[B.inv_rec[k] - rb] * bA[ops & 0xffff] * bV[ops >> 16];
             }
             ibar[sg.ibar + j] = qq * sf + g * ((Real(1) - qq) + Real(2) * qq * ef);
             Real ret = 0;
-            for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) {
-              uint32_t ops = B.rec_ops[k];
-              ret += coefs[k] * bV[ops & 0xffff] * bV[ops >> 16];
+            for (uint32_t k = B.it_rec[ia] - rb; k < B.it_rec[ia + 1] - rb; ++k) {
+              uint32_t ops = sOps[k];
+              ret += sCoef[k] * bV[ops & 0xffff] * bV[ops >> 16];
             }
             c0 = g * (ret - bV[i] + Real(2) * ef * bV[i]);  // -> q
             c1 = g * Real(2) * qq * bV[i];                   // -> ε of child0's top
@@ -838,9 +951,9 @@ __global__ void k_fill_f64(double* p, double v, size_t n) {
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-size_t lds_bytes(int mode, uint32_t cap) {
+size_t lds_bytes(int mode, uint32_t cap, uint32_t rcap) {
   size_t r = mode == WHALE_FP32_SCALED ? sizeof(float) : sizeof(double);
-  return 4 * (size_t)cap * r;
+  return (4 * (size_t)cap + rcap) * r + (size_t)rcap * sizeof(uint32_t);
 }
 
 template <typename K>
@@ -850,16 +963,17 @@ static hipError_t lds_attr(K kern, size_t lds) {
   return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, hipStream_t st, const BatchDev& B,
-                          const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
-                          uint32_t fam_end) {
-  size_t lds = lds_bytes(mode, cap);
+hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, uint32_t rcap, hipStream_t st,
+                          const BatchDev& B, const ModelDev& M, const WorkRaw& W, int32_t fam_single,
+                          uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count) {
+  size_t lds = lds_bytes(mode, cap, rcap);
   dim3 g(grid), b(kWG);
 #define WHALE_FWD(R, S, A)                                                                              \
   do {                                                                                                  \
     auto k = k_forward<R, S, A>;                                                                        \
     if (lds > 65536) { hipError_t e = lds_attr(k, lds); if (e != hipSuccess) return e; }               \
-    hipLaunchKernelGGL(k, g, b, lds, st, B, M, WorkDev<R>(W), cap, fam_single, fam_begin, fam_end);     \
+    hipLaunchKernelGGL(k, g, b, lds, st, B, M, WorkDev<R>(W), cap, rcap, fam_single, fam_begin, fam_end,    \
+                       fam_count);                                                                 \
   } while (0)
   if (mode == WHALE_FP64_PLAIN) { if (store_all) WHALE_FWD(double, false, true); else WHALE_FWD(double, false, false); }
   else if (mode == WHALE_FP32_SCALED) { if (store_all) WHALE_FWD(float, true, true); else WHALE_FWD(float, true, false); }
@@ -868,15 +982,17 @@ hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, hipS
   return hipGetLastError();
 }
 
-hipError_t launch_backward(int mode, int grid, uint32_t cap, hipStream_t st, const BatchDev& B, const ModelDev& M,
-                           const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end) {
-  size_t lds = lds_bytes(mode, cap);
+hipError_t launch_backward(int mode, int grid, uint32_t cap, uint32_t rcap, hipStream_t st, const BatchDev& B,
+                           const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
+                           uint32_t fam_end, const uint32_t* fam_count) {
+  size_t lds = lds_bytes(mode, cap, rcap);
   dim3 g(grid), b(kWG);
 #define WHALE_BWD(R, S)                                                                                 \
   do {                                                                                                  \
     auto k = k_backward<R, S>;                                                                          \
     if (lds > 65536) { hipError_t e = lds_attr(k, lds); if (e != hipSuccess) return e; }               \
-    hipLaunchKernelGGL(k, g, b, lds, st, B, M, WorkDev<R>(W), cap, fam_single, fam_begin, fam_end);     \
+    hipLaunchKernelGGL(k, g, b, lds, st, B, M, WorkDev<R>(W), cap, rcap, fam_single, fam_begin, fam_end,    \
+                       fam_count);                                                                 \
   } while (0)
   if (mode == WHALE_FP64_PLAIN) WHALE_BWD(double, false);
   else if (mode == WHALE_FP32_SCALED) WHALE_BWD(float, true);

@@ -51,6 +51,7 @@ struct FamDev {
   uint32_t lvl0;         // root size-level boundaries lvl[lvl0 .. lvl0 + n_lvl] (root-stage-local items)
   uint32_t n_lvl;
   uint32_t max_items;    // largest stage, root included
+  uint32_t max_recs;     // most records of any non-root stage
   uint32_t root_seg;     // absolute segment of the root
   uint64_t ell0;         // first ℓ cell (absolute)
   uint64_t ibar0;        // first ī cell (absolute)
