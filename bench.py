@@ -124,6 +124,9 @@ def main():
     batch = w.device_batch(model, ccds)
     st = batch.stats()
     log(f"parsed and planned in {time.time() - t0:.1f}s: {st}")
+    for r in batch.launches("fp32"):
+        log("launch fp32 {kernel} class {class}: {families} families, {per_workgroup}/workgroup, tables in LDS "
+            "{tables_in_lds}, grid {grid}, LDS {lds_bytes} B (cap {cap}, rcap {rcap})".format(**r))
     n_fam = st["families"]
     ctx = w.context()
     h = model.s.handle()

@@ -142,6 +142,13 @@ int whale_batch_stats(const whale_batch* b, int64_t* out12);
  * their FP32 values left the representable range.  No reference counterpart. */
 int whale_batch_fallbacks(const whale_batch* b, int32_t* n);
 
+/* Launch shapes of the batched kernels for `mode`: for the forward and the
+ * reverse sweep of each of the three size classes (6 rows of 8 values):
+ * [families, families per workgroup, slice tables in LDS (0/1), grid,
+ *  LDS bytes per workgroup, items per stage (cap), records per stage (rcap),
+ *  LDS bytes per family].  No reference counterpart (introspection). */
+int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* out48);
+
 /* logpdf of every family (unconditioned, src/core.jl:31-43) into
  * out_family[n_families] (may be NULL) and the conditioned batch value
  * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */

@@ -172,25 +172,43 @@ def test_discretization_convergence(w):
         assert np.all(np.abs(vals - vals[0]) < 0.1)
 
 
-def test_fp32_fallback_for_huge_paralog_expansion(w):
-    """64 in-paralogs on a long, finely sliced branch: the clade values grow far
-    beyond FP32's range after their exponents are fixed; such families are
-    recomputed in FP64 with per-clade exponents."""
+def _paralog_case(w, tree, dt, others, lam, mu):
     from ale_util import ale_from_tree, balanced_paralogs
-    adj, names = balanced_paralogs(64, "A", ("B_1", "B_2"))
+    adj, names = balanced_paralogs(64, "A", others)
     text = ale_from_tree(adj, names)
-    tree = "(A:1.0,B:1.0);"
-    om = OracleModel(tree, [], 0.005)
+    om = OracleModel(tree, [], dt)
     oc = om.ccds([text])
-    r = ODLWGD([np.log(0.8)] * 3, [np.log(0.3)] * 3, [], 0.9)
-    ref = om.logpdf(oc, r, per_family=True, precision="long")[0]
-    m = w.WhaleModel(w.DLWGD(λ=[np.log(0.8)] * 3, μ=[np.log(0.3)] * 3, η=0.9), w.readnw(tree), 0.005)
+    ref = om.logpdf(oc, ODLWGD(list(lam), list(mu), [], 0.9), per_family=True, precision="long")[0]
+    m = w.WhaleModel(w.DLWGD(λ=list(lam), μ=list(mu), η=0.9), w.readnw(tree), dt)
     c = w.ccds_from_texts([text], m)
     got = w.logpdf_families(m, c, mode="fp32")[0]
-    b = w.device_batch(m, c)
-    print("huge paralog family: ref", ref, "fp32", got, "recomputed in fp64:", b.fallbacks())
-    assert b.fallbacks() == 1
-    assert abs(got - ref) <= 1e-9 * abs(ref)
+    n_fb = w.device_batch(m, c).fallbacks()
     v, g, _ = w.gradient(m, c, mode="fp32")
     v64, g64, _ = w.gradient(m, c, mode="fp64s")
+    return ref, got, n_fb, g, g64
+
+
+def test_fp32_young_clade_renormalised_in_place(w):
+    """64 in-paralogs on one long, finely sliced leaf branch: their clade values
+    grow by ~2^360 after first becoming non-zero.  They had no values before
+    this branch, so FP32 renormalises them in place; no FP64 recomputation."""
+    ref, got, n_fb, g, g64 = _paralog_case(w, "(A:1.0,B:1.0);", 0.005, ("B_1", "B_2"),
+                                           [np.log(0.8)] * 3, [np.log(0.3)] * 3)
+    print("young expansion: ref", ref, "fp32", got, "fallbacks", n_fb)
+    assert n_fb == 0
+    assert abs(got - ref) <= 1e-6 * abs(ref)
+    assert np.max(np.abs(g - g64)) <= 1e-5 * np.max(np.abs(g64))
+
+
+def test_fp32_fallback_recomputes_in_fp64(w):
+    """The same clade born at the top of a 6-slice leaf branch grows by ~2^354
+    along a 200-slice parent branch (its sister loses genes readily): FP32
+    cannot hold it, the family is recomputed in FP64 with per-clade exponents."""
+    mu = [np.log(0.3)] * 5
+    mu[1] = np.log(3.0)  # node 2 (B)
+    ref, got, n_fb, g, g64 = _paralog_case(w, "((A:0.03,B:1.0):1.0,C:1.03);", 0.005, ("B_1", "C_1"),
+                                           [np.log(0.8)] * 5, mu)
+    print("grown expansion: ref", ref, "fp32", got, "fallbacks", n_fb)
+    assert n_fb == 1
+    assert abs(got - ref) <= 1e-9 * abs(ref)
     assert np.max(np.abs(g - g64)) <= 1e-9 * np.max(np.abs(g64))

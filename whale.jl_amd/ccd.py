@@ -139,6 +139,19 @@ class DeviceBatch:
         check(_lib.lib().whale_batch_fallbacks(self.h, C.byref(n)), "whale_batch_fallbacks")
         return n.value
 
+    def launches(self, mode: str = "fp32") -> list:
+        """Launch shape of each batched kernel: forward and reverse sweep per size class."""
+        o = np.zeros(48, np.int64)
+        check(_lib.lib().whale_batch_launches(self.h, _lib.MODES[mode], ptr(o, i64p)), "whale_batch_launches")
+        keys = ["families", "per_workgroup", "tables_in_lds", "grid", "lds_bytes", "cap", "rcap", "family_lds_bytes"]
+        out = []
+        for d, name in enumerate(("forward", "backward")):
+            for k in range(3):
+                r = dict(zip(keys, (int(x) for x in o[8 * (3 * d + k):8 * (3 * d + k + 1)])))
+                r["kernel"], r["class"] = name, k
+                out.append(r)
+        return out
+
     def ell(self, fam: int) -> np.ndarray:
         n = C.c_int64()
         check(_lib.lib().whale_download_ell(self.h, fam, None, 0, C.byref(n)), "whale_download_ell")

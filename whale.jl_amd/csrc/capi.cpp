@@ -59,10 +59,19 @@ hipError_t upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 }  // namespace
 
+struct whale_batch;
+namespace {
+int plan_batch_launches(whale_batch* b);
+}  // namespace
+
 struct whale_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side[2] = {};       // size classes run beside each other on these and `stream`
+  hipEvent_t fork[6] = {};
   int n_cu = 256;
+  size_t lds_wg = 64 * 1024;   // LDS one workgroup may allocate
+  size_t lds_cu = 64 * 1024;   // LDS of one compute unit
   bool timing = false;
   hipEvent_t ev[6] = {};
   hipEvent_t user[16] = {};
@@ -106,8 +115,14 @@ struct whale_batch {
   // of families that left the FP32 range after the first align8(4 * n) bytes.
   DBuf ell, coef, E, ibar, fam_ll, acc, ticket, flag, fb;
   uint64_t n_rec = 0;
-  uint32_t n_fam = 0, n_large = 0;
-  uint32_t cap_small = 0, cap_large = 0, rcap_small = 0, rcap_large = 0;
+  uint32_t n_fam = 0;
+  // size classes: families order[cls_lo[c] .. cls_hi[c]) hold stages of at most
+  // cls_cap[c] items and cls_rcap[c] records (class 2: the batch maximum)
+  static constexpr int kClasses = 3;
+  uint32_t cls_lo[kClasses] = {}, cls_hi[kClasses] = {};
+  uint32_t cls_cap[kClasses] = {}, cls_rcap[kClasses] = {};
+  LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
+  LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   int last_mode = -1;
   bool last_store_all = false;
   uint32_t last_fallback = 0;  // families of the last FP32 call recomputed in FP64
@@ -182,9 +197,19 @@ int whale_ctx_create(int device, whale_ctx** out) {
   auto c = std::make_unique<whale_ctx>();
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (auto& sd : c->side) HIPCHK(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+  for (auto& e : c->fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
   c->n_cu = prop.multiProcessorCount;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+      c->lds_wg = (size_t)v;
+    c->lds_cu = c->lds_wg;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess && v > 0)
+      c->lds_cu = std::max(c->lds_wg, (size_t)v);
+  }
   for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
   for (auto& e : c->user) HIPCHK(hipEventCreate(&e));
   *out = c.release();
@@ -196,6 +221,8 @@ int whale_ctx_destroy(whale_ctx* c) {
   (void)hipSetDevice(c->device);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto& e : c->user) if (e) (void)hipEventDestroy(e);
+  for (auto& e : c->fork) if (e) (void)hipEventDestroy(e);
+  for (auto& sd : c->side) if (sd) (void)hipStreamDestroy(sd);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return WHALE_OK;
@@ -476,19 +503,37 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   if (P.rec_ops.size() > 0xffffffffull || P.inv_ops.size() > 0xffffffffull || P.it_seg.size() > 0xffffffffull)
     return fail(WHALE_EINVAL, "batch too large");
   b->n_fam = (uint32_t)n_fam;
-  // size classes: families whose largest stage fits the small LDS cap run in a
-  // launch with many workgroups per CU; the rest get their own launch.
-  const uint32_t small_cap = 512, small_rcap = 1024;
-  std::vector<uint32_t> large, small;
-  for (uint32_t f : P.cost_order)
-    (P.fam[f].max_items > small_cap || P.fam[f].max_recs > small_rcap ? large : small).push_back(f);
-  b->order_host = large;
-  b->order_host.insert(b->order_host.end(), small.begin(), small.end());
-  b->n_large = (uint32_t)large.size();
-  b->cap_small = small_cap;
-  b->rcap_small = small_rcap;
-  b->cap_large = std::max(P.max_items, 1u);
-  b->rcap_large = std::max(P.max_recs, 2u);
+  // size classes: families whose stages (root included) fit a small LDS budget
+  // run many to a compute unit.  Order: large class first, then small, then
+  // tiny; within a class the costliest families first.
+  {
+    const uint32_t caps[2] = {256, 512}, rcaps[2] = {512, 1024};
+    std::vector<uint32_t> cls[whale_batch::kClasses];
+    for (uint32_t f : P.cost_order) {
+      const FamDev& d = P.fam[f];
+      const uint32_t recs = std::max(d.max_recs, d.root_recs);
+      int k = 2;
+      for (int j = 1; j >= 0; --j)
+        if (d.max_items <= caps[j] && recs <= rcaps[j]) k = j;
+      cls[k].push_back(f);
+    }
+    b->order_host.clear();
+    for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
+      b->cls_lo[k] = (uint32_t)b->order_host.size();
+      b->order_host.insert(b->order_host.end(), cls[k].begin(), cls[k].end());
+      b->cls_hi[k] = (uint32_t)b->order_host.size();
+    }
+    for (int j = 0; j < 2; ++j) {
+      b->cls_cap[j] = caps[j];
+      b->cls_rcap[j] = rcaps[j];
+    }
+    b->cls_cap[2] = std::max(P.max_items, 16u);
+    b->cls_rcap[2] = std::max(std::max(P.max_recs, P.max_root_recs), 4u);
+  }
+  {
+    const int rc = plan_batch_launches(b.get());
+    if (rc != WHALE_OK) return rc;
+  }
   whale_ctx* c = m->ctx;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
@@ -552,6 +597,25 @@ int whale_batch_fallbacks(const whale_batch* b, int32_t* n) {
   return WHALE_OK;
 }
 
+int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* o) {
+  if (!b || !o) return fail(WHALE_EINVAL, "NULL argument");
+  if (mode < 0 || mode > 2) return fail(WHALE_EINVAL, "unknown mode");
+  for (int dir = 0; dir < 2; ++dir)
+    for (int k = 0; k < whale_batch::kClasses; ++k) {
+      const LaunchCfg& C = b->cfg[mode][dir][k];
+      int64_t* r = o + 8 * (3 * dir + k);
+      r[0] = b->cls_hi[k] - b->cls_lo[k];
+      r[1] = C.fpw;
+      r[2] = C.tabl ? 1 : 0;
+      r[3] = C.grid;
+      r[4] = (int64_t)C.lds();
+      r[5] = C.L.cap;
+      r[6] = C.L.rcap;
+      r[7] = C.L.wave_bytes;
+    }
+  return WHALE_OK;
+}
+
 int whale_batch_stats(const whale_batch* b, int64_t* o) {
   if (!b || !o) return fail(WHALE_EINVAL, "NULL argument");
   const PlanHost& P = b->plan;
@@ -572,12 +636,76 @@ int whale_batch_stats(const whale_batch* b, int64_t* o) {
 
 namespace {
 
-int grid_for(const whale_batch* b, int mode, uint32_t cap, uint32_t rcap, uint32_t nfam) {
-  // workgroups resident per CU: LDS-limited, at most 16 one-wave workgroups
-  size_t lds = std::max<size_t>(1, lds_bytes(mode, cap, rcap));
-  int per_cu = (int)std::min<size_t>(16, std::max<size_t>(1, (160 * 1024) / lds));
-  int g = b->model->ctx->n_cu * per_cu;
-  return (int)std::min<uint32_t>((uint32_t)g, std::max<uint32_t>(1u, nfam));
+// Launch shape of one forward or backward launch over families whose stages
+// hold at most `cap` items and `rcap` records.  The slice tables go to LDS
+// whenever they fit; then the shape with the most resident waves per compute
+// unit wins (LDS-limited, at most kWavesPerCU).
+constexpr int kWavesPerCU = 16;
+
+bool plan_launch(const whale_model* m, bool fwd, int mode, uint32_t cap, uint32_t rcap, uint64_t nfam, bool multi,
+                 LaunchCfg* out) {
+  cap = (std::max(cap, 1u) + 15) & ~15u;
+  rcap = (std::max(rcap, 4u) + 3) & ~3u;
+  const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+  const size_t wave = fwd ? fwd_wave_bytes(mode, cap, rcap) : bwd_wave_bytes(mode, cap, rcap);
+  const size_t shared = fwd ? fwd_shared_bytes(mode, m->h.tab_rows) : bwd_shared_bytes(mode, m->h.tab_rows, AL.n);
+  struct Cand {
+    int fpw;
+    bool tabl;
+  } const cands[3] = {{4, true}, {1, true}, {1, false}};
+  int best = -1, best_waves = 0, best_wg = 0;
+  for (int j = 0; j < 3; ++j) {
+    if (cands[j].fpw > 1 && !multi) continue;
+    if (best >= 0 && cands[best].tabl && !cands[j].tabl) break;
+    const size_t lds = (cands[j].tabl ? shared : 0) + (size_t)cands[j].fpw * wave;
+    if (lds > m->ctx->lds_wg) continue;
+    const int wg = (int)std::min<size_t>(m->ctx->lds_cu / lds, (size_t)(kWavesPerCU / cands[j].fpw));
+    if (wg * cands[j].fpw > best_waves) {
+      best = j;
+      best_waves = wg * cands[j].fpw;
+      best_wg = wg;
+    }
+  }
+  if (best < 0) return false;
+  LaunchCfg C;
+  C.fpw = cands[best].fpw;
+  C.tabl = cands[best].tabl;
+  C.L.cap = cap;
+  C.L.rcap = rcap;
+  C.L.wave_bytes = (uint32_t)wave;
+  C.L.shared_bytes = C.tabl ? (uint32_t)shared : 0u;
+  const uint64_t need = (nfam + C.fpw - 1) / C.fpw;
+  C.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)m->ctx->n_cu * best_wg, need));
+  *out = C;
+  return true;
+}
+
+// launch shapes of every class, mode and direction, and of the FP64 pass over
+// families the FP32 form could not represent
+int plan_batch_launches(whale_batch* b) {
+  const whale_model* m = b->model;
+  const PlanHost& P = b->plan;
+  for (int mode = 0; mode < 3; ++mode)
+    for (int dir = 0; dir < 2; ++dir)
+      for (int k = 0; k < whale_batch::kClasses; ++k) {
+        const uint64_t nf = b->cls_hi[k] - b->cls_lo[k];
+        uint32_t rcap = b->cls_rcap[k];
+        bool ok = plan_launch(m, dir == 0, mode, b->cls_cap[k], rcap, nf, true, &b->cfg[mode][dir][k]);
+        if (!ok && k == 2) {  // root stages then read their records from global memory
+          rcap = std::max(P.max_recs, 4u);
+          ok = plan_launch(m, dir == 0, mode, b->cls_cap[k], rcap, nf, true, &b->cfg[mode][dir][k]);
+        }
+        if (!ok) return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
+      }
+  for (int dir = 0; dir < 2; ++dir) {
+    bool ok = plan_launch(m, dir == 0, WHALE_FP64_SCALED, b->cls_cap[2], b->cls_rcap[2], (uint64_t)m->ctx->n_cu, false,
+                          &b->cfg_fb[dir]);
+    if (!ok)
+      ok = plan_launch(m, dir == 0, WHALE_FP64_SCALED, b->cls_cap[2], std::max(P.max_recs, 4u),
+                       (uint64_t)m->ctx->n_cu, false, &b->cfg_fb[dir]);
+    if (!ok) return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
+  }
+  return WHALE_OK;
 }
 
 // All device work of one evaluation on the context's stream (capturable: no
@@ -593,7 +721,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
   HIPCHK(launch_fill_u32(st, W.ticket, 0u, 8));
   if (grad) HIPCHK(launch_fill_f64(st, W.acc, 0.0, AL.n));
-  const uint32_t nl = b->n_large, nf = b->n_fam;
+  const uint32_t nf = b->n_fam;
   const bool fp32 = mode == WHALE_FP32_SCALED;
   if (fp32) {
     HIPCHK(launch_fill_u32(st, W.flag, 0u, (int)nf));
@@ -603,53 +731,63 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   BatchDev Bfb = B;
   Bfb.order = W.fb + 1;
   WorkRaw Wfb = b->work_fallback();
-  Wfb.ticket = W.ticket + 4;
-  const int fb_grid = std::max(1, c->n_cu);
+  // size classes run concurrently: class 0 on the context's stream, 1 and 2 on the side streams
+  auto batched = [&](int dir) -> int {
+    int used = 0;
+    for (int k = 0; k < whale_batch::kClasses; ++k) used += b->cls_hi[k] > b->cls_lo[k];
+    const bool fork = used > 1;
+    const int e0 = 3 * dir;
+    if (fork) HIPCHK(hipEventRecord(c->fork[e0], st));
+    for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
+      if (b->cls_hi[k] == b->cls_lo[k]) continue;
+      hipStream_t sk = (fork && k > 0) ? c->side[k - 1] : st;
+      if (sk != st) HIPCHK(hipStreamWaitEvent(sk, c->fork[e0], 0));
+      WorkRaw Wk = W;
+      Wk.ticket = W.ticket + 3 * dir + k;
+      const LaunchCfg& C = b->cfg[mode][dir][k];
+      if (dir == 0) HIPCHK(launch_forward(mode, store_all, C, sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
+      else HIPCHK(launch_backward(mode, C, sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
+      if (sk != st) HIPCHK(hipEventRecord(c->fork[e0 + k], sk));
+    }
+    if (fork)
+      for (int k = 1; k < whale_batch::kClasses; ++k)
+        if (b->cls_hi[k] > b->cls_lo[k]) HIPCHK(hipStreamWaitEvent(st, c->fork[e0 + k], 0));
+    return WHALE_OK;
+  };
+  auto single = [&](int dir, uint32_t f, const WorkRaw& Wf) -> int {
+    const FamDev& fd = b->plan.fam[f];
+    LaunchCfg C;
+    if (!plan_launch(m, dir == 0, mode, fd.max_items, std::max(fd.max_recs, fd.root_recs), 1, false, &C) &&
+        !plan_launch(m, dir == 0, mode, fd.max_items, fd.max_recs, 1, false, &C))
+      return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
+    C.grid = 1;
+    if (dir == 0) HIPCHK(launch_forward(mode, store_all, C, st, B, M, Wf, (int32_t)f, 0, 0));
+    else HIPCHK(launch_backward(mode, C, st, B, M, Wf, (int32_t)f, 0, 0));
+    return WHALE_OK;
+  };
+  int rc;
   if (launch == WHALE_LAUNCH_BATCHED) {
-    if (nl)
-      HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_large, b->rcap_large, nl), b->cap_large,
-                            b->rcap_large, st, B, M, W, -1, 0, nl));
-    if (nf > nl) {
-      WorkRaw W2 = W;
-      W2.ticket = W.ticket + 1;
-      HIPCHK(launch_forward(mode, store_all, grid_for(b, mode, b->cap_small, b->rcap_small, nf - nl), b->cap_small,
-                            b->rcap_small, st, B, M, W2, -1, nl, nf));
-    }
+    if ((rc = batched(0)) != WHALE_OK) return rc;
   } else {
-    for (uint32_t k = 0; k < nf; ++k) {
-      uint32_t f = b->order_host[k];
-      const FamDev& fd = b->plan.fam[f];
-      HIPCHK(launch_forward(mode, store_all, 1, fd.max_items, std::max(fd.max_recs, 2u), st, B, M, W, (int32_t)f, 0, 0));
-    }
+    for (uint32_t k = 0; k < nf; ++k)
+      if ((rc = single(0, b->order_host[k], W)) != WHALE_OK) return rc;
   }
-  if (fp32)
-    HIPCHK(launch_forward(WHALE_FP64_SCALED, store_all, fb_grid, b->cap_large, b->rcap_large, st, Bfb, M, Wfb, -1, 0,
-                          0, W.fb));
+  if (fp32) {
+    Wfb.ticket = W.ticket + 6;
+    HIPCHK(launch_forward(WHALE_FP64_SCALED, store_all, b->cfg_fb[0], st, Bfb, M, Wfb, -1, 0, 0, W.fb));
+  }
   if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
   if (grad) {
     if (launch == WHALE_LAUNCH_BATCHED) {
-      WorkRaw W3 = W;
-      W3.ticket = W.ticket + 2;
-      if (nl)
-        HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_large, b->rcap_large, nl), b->cap_large, b->rcap_large,
-                               st, B, M, W3, -1, 0, nl));
-      if (nf > nl) {
-        W3.ticket = W.ticket + 3;
-        HIPCHK(launch_backward(mode, grid_for(b, mode, b->cap_small, b->rcap_small, nf - nl), b->cap_small,
-                               b->rcap_small, st, B, M, W3, -1, nl, nf));
-      }
+      if ((rc = batched(1)) != WHALE_OK) return rc;
     } else {
-      for (uint32_t k = 0; k < nf; ++k) {
-        uint32_t f = b->order_host[k];
-        const FamDev& fd = b->plan.fam[f];
-        HIPCHK(launch_backward(mode, 1, fd.max_items, std::max(fd.max_recs, 2u), st, B, M, W, (int32_t)f, 0, 0));
-      }
+      for (uint32_t k = 0; k < nf; ++k)
+        if ((rc = single(1, b->order_host[k], W)) != WHALE_OK) return rc;
     }
     if (fp32) {
       WorkRaw Wfb2 = Wfb;
-      Wfb2.ticket = W.ticket + 5;
-      HIPCHK(launch_backward(WHALE_FP64_SCALED, fb_grid, b->cap_large, b->rcap_large, st, Bfb, M, Wfb2, -1, 0, 0,
-                             W.fb));
+      Wfb2.ticket = W.ticket + 7;
+      HIPCHK(launch_backward(WHALE_FP64_SCALED, b->cfg_fb[1], st, Bfb, M, Wfb2, -1, 0, 0, W.fb));
     }
   }
   if (timed) HIPCHK(hipEventRecord(c->ev[3], st));

@@ -70,18 +70,43 @@ struct AccLayout {
   }
 };
 
+__host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+// LDS of one launch of k_forward / k_backward: every wave of a workgroup runs
+// its own family in `wave_bytes` of LDS holding `cap` items and `rcap` records
+// of a stage; `shared_bytes` (slice tables, reverse-sweep accumulators) come
+// first and are shared by the workgroup's waves.
+struct LdsCfg {
+  uint32_t cap, rcap;
+  uint32_t wave_bytes;
+  uint32_t shared_bytes;
+};
+
+// a launch: `fpw` families (waves) per workgroup, slice tables in LDS or not
+struct LaunchCfg {
+  int grid = 1;
+  int fpw = 1;
+  bool tabl = false;
+  LdsCfg L{};
+  size_t lds() const { return (size_t)L.shared_bytes + (size_t)fpw * L.wave_bytes; }
+};
+
+size_t fwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap);
+size_t bwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap);
+size_t fwd_shared_bytes(int mode, int tab_rows);
+size_t bwd_shared_bytes(int mode, int tab_rows, int acc_n);
+
 // forward/backward launch: `mode` is WHALE_FP64_PLAIN / WHALE_FP32_SCALED /
-// WHALE_FP64_SCALED; fam_single >= 0 runs that one family on a 1-block grid,
-// otherwise blocks take families order[fam_begin .. fam_end) from ticket.
-// `cap` is the LDS capacity in items, `rcap` in records (of any non-root stage).
-// fam_count, if not null, replaces fam_end with a count read on the device
-// (the FP64 fallback pass over families listed by the FP32 pass).
-hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, uint32_t rcap, hipStream_t st,
-                          const BatchDev& B, const ModelDev& M, const WorkRaw& W, int32_t fam_single,
-                          uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count = nullptr);
-hipError_t launch_backward(int mode, int grid, uint32_t cap, uint32_t rcap, hipStream_t st, const BatchDev& B,
-                           const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
-                           uint32_t fam_end, const uint32_t* fam_count = nullptr);
+// WHALE_FP64_SCALED; fam_single >= 0 runs that one family (first wave of a
+// 1-block grid), otherwise waves take families order[fam_begin .. fam_end)
+// from the ticket.  fam_count, if not null, replaces fam_end with a count read
+// on the device (the FP64 pass over families listed by the FP32 pass).
+hipError_t launch_forward(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
+                          const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
+                          uint32_t fam_end, const uint32_t* fam_count = nullptr);
+hipError_t launch_backward(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                           const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
+                           const uint32_t* fam_count = nullptr);
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
                          double* dnode, const int32_t* hlevels, const int32_t* hnodes, int n_levels);
 hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,
@@ -89,6 +114,5 @@ hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, 
                        const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int n);
 hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n);
-size_t lds_bytes(int mode, uint32_t cap, uint32_t rcap);
 
 }  // namespace whale

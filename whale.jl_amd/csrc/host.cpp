@@ -212,7 +212,7 @@ namespace {
 
 struct FamPlan {
   std::string err;
-  uint32_t n_clades = 0, max_items = 0, max_recs = 0;
+  uint32_t n_clades = 0, max_items = 0, max_recs = 0, root_recs = 0;
   std::vector<StageDev> stage;      // item0 relative to family, seg0 relative
   std::vector<SegDev> seg;          // ell/ibar relative to family
   uint32_t root_seg = 0, root_item0 = 0;
@@ -393,9 +393,10 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
         }
       }
     }
-    if (!is_root) {
+    {
       uint32_t nrec = (uint32_t)F.rec_ops.size() - F.it_rec[sd.item0];
-      F.max_recs = std::max(F.max_recs, nrec);
+      if (is_root) F.root_recs = nrec;
+      else F.max_recs = std::max(F.max_recs, nrec);
     }
     for (uint32_t i = 0; i < local; ++i) {
       F.it_inv.push_back((uint32_t)F.inv_ops.size());
@@ -488,6 +489,7 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
     d.n_lvl = (uint32_t)F.lvl.size() - 1;
     d.max_items = F.max_items;
     d.max_recs = F.max_recs;
+    d.root_recs = F.root_recs;
     d.root_seg = seg_base + F.root_seg;
     d.ell0 = ell;
     d.ibar0 = ibar;
@@ -518,6 +520,7 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
     ibar += F.ibar;
     P.max_items = std::max(P.max_items, F.max_items);
     P.max_recs = std::max(P.max_recs, F.max_recs);
+    P.max_root_recs = std::max(P.max_root_recs, F.root_recs);
     P.cells += F.cells;
     P.split_terms += F.split_terms;
     P.sum_compat += F.sum_compat;

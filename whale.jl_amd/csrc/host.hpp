@@ -51,6 +51,7 @@ struct PlanHost {
   uint64_t ell_cells = 0, ibar_cells = 0;
   uint32_t max_items = 0;
   uint32_t max_recs = 0;             // most records of any non-root stage
+  uint32_t max_root_recs = 0;        // most records of a root stage
   // workload counts
   int64_t n_clades = 0, n_splits = 0, cells = 0, split_terms = 0, sum_compat = 0, row_steps = 0;
   // per-family compat lists in the reference layout, for ℓ download:

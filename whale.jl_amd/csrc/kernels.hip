@@ -38,6 +38,20 @@ constexpr int kWG = 64;              // one wave per family
 // growth and 2^-(126-kBirth) of decay before FP32 runs out of range; a family
 // that still leaves the range is recomputed in FP64 (see k_forward's flag).
 constexpr int kBirth = 40;
+// A clade that had no value before the current stage ("young") may still grow
+// by many powers of two within this stage's branch (a large in-paralog clade
+// gains a factor per slice as more slices hold its duplications).  Because it
+// has no stored values elsewhere, its exponent can be raised in place: when
+// its FP32 value passes 2^kBump the clade is renormalised and the rows it
+// already wrote on this branch are rescaled.  The window [2^-kBirth, 2^kBump]
+// is kept narrow because a split's coefficient p·2^(E1+E2-E) spans the ratio
+// of the parent's value to the product of its parts' values (two windows and
+// one 1/ψ): 2^(kBump + 2·kBirth + log2(1/ψ)) must stay inside FP32.  A narrow
+// window also ends the stage low, with room to grow in the stages above.
+constexpr int kBump = 16;
+constexpr uint8_t kPending = 1;  // exponent still open (all values so far are zero)
+constexpr uint8_t kYoung = 2;    // exponent was open when the stage began
+constexpr uint8_t kShifted = 4;  // renormalised in this row; exponent update pending
 
 template <typename Real>
 struct WorkDev {
@@ -183,171 +197,218 @@ __global__ void k_tables(ModelDev M, double* eps, double* phi, double* psi, doub
 }
 
 // ---------------------------------------------------------------------------
-// wave-level segmented sum: items of one segment sit on consecutive lanes
+// wave-level reductions
 // ---------------------------------------------------------------------------
+// Segmented inclusive sums of two values over runs of equal keys on
+// consecutive lanes; the last lane of a run holds the run's sums (*tail).
 template <typename Real>
-__device__ __forceinline__ void seg_add(Real v, uint32_t key, double* addr) {
-  const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void seg_scan2(Real& v1, Real& v2, uint32_t key, bool* tail) {
+  const int lane = threadIdx.x & (kWG - 1);
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    Real vu = __shfl_up(v, off, 64);
-    uint32_t ku = __shfl_up(key, off, 64);
-    if (lane >= off && ku == key) v += vu;
+  for (int off = 1; off < kWG; off <<= 1) {
+    const Real u1 = __shfl_up(v1, off, kWG);
+    const Real u2 = __shfl_up(v2, off, kWG);
+    const uint32_t ku = __shfl_up(key, off, kWG);
+    if (lane >= off && ku == key) {
+      v1 += u1;
+      v2 += u2;
+    }
   }
-  uint32_t kn = __shfl_down(key, 1, 64);
-  bool tail = (lane == 63) || (kn != key);
-  if (tail && key != kNone && v != Real(0)) unsafeAtomicAdd(addr, (double)v);
+  const uint32_t kn = __shfl_down(key, 1, kWG);
+  *tail = (lane == kWG - 1) || (kn != key);
 }
 
 template <typename Real>
 __device__ __forceinline__ Real wave_sum(Real v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWG);
   return v;
 }
 
-// coefficient of record r in the scaled forms: p·2^(E1+E2-E), 0 while any exponent is open
+// atomic adds without return: LDS (ds_add) or global (FP64 atomics in L2)
 template <typename Real>
-__device__ __forceinline__ Real scaled_coef(const BatchDev& B, const FamDev& F, const int32_t* E, uint32_t item0,
-                                            uint32_t rec, int Eg) {
-  uint32_t ops = B.rec_ops[rec];
-  int e1 = E[F.clade0 + B.it_gid[item0 + (ops & 0xffff)]];
-  int e2 = E[F.clade0 + B.it_gid[item0 + (ops >> 16)]];
-  if (Eg == kUndef || e1 == kUndef || e2 == kUndef) return Real(0);
-  return ldexp_r((Real)B.split_p[F.split0 + B.rec_sid[rec]], e1 + e2 - Eg);
+__device__ __forceinline__ void lds_add(Real* p, Real v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <bool Lds>
+__device__ __forceinline__ void acc_add(double* p, double v) {
+  if (v == 0.0) return;
+  if (Lds) lds_add(p, v);
+  else unsafeAtomicAdd(p, v);
 }
 
-// Every workgroup is a single wavefront (kWG == 64).  A wave issues its LDS
-// instructions in order and the LDS executes them in order, so between the
-// rows of a stage it is enough to stop the compiler from moving LDS accesses
-// across the row boundary; no s_barrier and no wait on outstanding global
-// stores is needed.  Stage boundaries (where global ℓ written earlier is read
-// back) keep __syncthreads().
-static_assert(kWG == 64, "row synchronisation assumes one wavefront per workgroup");
+// coefficient of a record in the scaled forms: p·2^(E1+E2-E), 0 while any exponent is open
+template <typename Real>
+__device__ __forceinline__ Real coef_of(Real p, int e1, int e2, int Eg) {
+  if (Eg == kUndef || e1 == kUndef || e2 == kUndef) return Real(0);
+  return ldexp_r(p, e1 + e2 - Eg);
+}
+
+// One wavefront evaluates one family; a workgroup holds FPW such waves, each
+// on its own family, sharing the slice tables (and, in the reverse sweep, the
+// gradient accumulators) in LDS.  All synchronisation inside a family is
+// therefore wave-level.  The LDS executes a wave's instructions in order, so
+// between rows it is enough to keep the compiler from moving LDS accesses
+// across the boundary: no s_barrier and no wait on outstanding global stores.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Where a wave reads back global memory its lanes wrote (ℓ and ī of earlier
+// stages, exponents) the stores must have completed: once per stage.
+__device__ __forceinline__ void wave_sync_mem() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// φ_r, ψ_r of table row `row`: from the workgroup's LDS copy or from global memory
+template <typename Real, bool TabL>
+__device__ __forceinline__ void slice(const ModelDev& M, const Real* sTab, uint32_t row, Real& ph, Real& ps) {
+  if (TabL) {
+    typedef Real Real2 __attribute__((ext_vector_type(2)));
+    const Real2 t = reinterpret_cast<const Real2*>(sTab)[row];
+    ph = t.x;
+    ps = t.y;
+  } else {
+    ph = (Real)M.phi[row];
+    ps = (Real)M.psi[row];
+  }
+}
 
 // ---------------------------------------------------------------------------
 // row-invariant data of one item, cached in registers for a stage's rows
 // ---------------------------------------------------------------------------
-constexpr int kCache = 4;  // items per lane held in registers (stages of up to 256 items)
+constexpr int kCache = 8;  // items per lane held in registers (stages of up to 512 items)
 
 struct ItemRow {
   uint32_t cell;    // ℓ cell of row 0, relative to the family
   uint32_t stride;  // |C_e| of the item's segment (row pitch)
   uint32_t tab;     // first table row of the segment's node
-  uint32_t r0, r1;  // records
-  uint32_t q0, q1;  // inverse records
   uint32_t R;       // rows of the segment
-  uint32_t gid;     // family-local clade id
-  uint32_t seg;     // absolute segment (reduction key)
 };
 
-__device__ __forceinline__ ItemRow load_item(const BatchDev& B, const ModelDev& M, uint32_t ia, uint32_t i) {
+__device__ __forceinline__ ItemRow load_item(const BatchDev& B, const ModelDev& M, uint32_t ia, uint32_t i,
+                                             const uint32_t* sRO) {
   ItemRow it;
-  it.seg = B.it_seg[ia];
-  const SegDev sg = B.seg[it.seg];
+  const SegDev sg = B.seg[B.it_seg[ia]];
   it.cell = sg.ell + (i - sg.item0);
   it.stride = sg.nc;
   it.tab = M.node[sg.node].tab;
-  it.r0 = B.it_rec[ia];
-  it.r1 = B.it_rec[ia + 1];
-  it.q0 = B.it_inv[ia];
-  it.q1 = B.it_inv[ia + 1];
   it.R = sg.R;
-  it.gid = B.it_gid[ia];
   return it;
 }
+
+// LDS of one wave (forward):  bA bB bF [cap] sCoef [rcap] (Real) | sOps [rcap] |
+// sRO [cap+1] (record offsets) | sEi [cap] (exponents) sFlag [cap] (scaled forms)
+template <typename Real, bool Scaled>
+struct FwdLds {
+  Real *bA, *bB, *bF, *sCoef;
+  uint32_t *sOps, *sRO;
+  int32_t* sEi;
+  uint8_t* sFlag;
+  __device__ FwdLds(unsigned char* wb, uint32_t cap, uint32_t rcap) {
+    bA = reinterpret_cast<Real*>(wb);
+    bB = bA + cap;
+    bF = bB + cap;
+    sCoef = bF + cap;
+    sOps = reinterpret_cast<uint32_t*>(wb + align16((3 * (size_t)cap + rcap) * sizeof(Real)));
+    sRO = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(sOps) + align16(4 * (size_t)rcap));
+    sEi = reinterpret_cast<int32_t*>(reinterpret_cast<unsigned char*>(sRO) + align16(4 * ((size_t)cap + 1)));
+    sFlag = reinterpret_cast<uint8_t*>(sEi + cap);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-template <typename Real, bool Scaled, bool StoreAll>
-__global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, uint32_t cap,
-                                                  uint32_t rcap, int32_t fam_single, uint32_t fam_begin,
-                                                  uint32_t fam_end, const uint32_t* fam_count) {
+template <typename Real, bool Scaled, bool StoreAll, int FPW, bool TabL>
+__global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+                                                       int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
+                                                       const uint32_t* fam_count) {
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
-  Real* bA = reinterpret_cast<Real*>(smem);
-  Real* bB = bA + cap;
-  Real* bF = bB + cap;
-  Real* bG = bF + cap;
-  Real* sCoef = bG + cap;                                    // rcap coefficients of the stage's records
-  uint32_t* sOps = reinterpret_cast<uint32_t*>(sCoef + rcap);  // rcap operand pairs
-  __shared__ uint32_t s_fam;
-  __shared__ int s_dirty;
-  const int lane = threadIdx.x;
-  const Real* coefs = Scaled ? W.coef : reinterpret_cast<const Real*>(B.rec_p);
+  const uint32_t lane = threadIdx.x % kWG;
+  const uint32_t wv = threadIdx.x / kWG;
+  Real* sTab = reinterpret_cast<Real*>(smem);
+  if (TabL) {
+    for (int r = threadIdx.x; r < M.tab_rows; r += kWG * FPW) {
+      sTab[2 * r] = (Real)M.phi[r];
+      sTab[2 * r + 1] = (Real)M.psi[r];
+    }
+    __syncthreads();
+  }
+  const uint32_t cap = L.cap, rcap = L.rcap;
+  FwdLds<Real, Scaled> S_(smem + L.shared_bytes + (size_t)wv * L.wave_bytes, cap, rcap);
+  Real* const bA = S_.bA;
+  Real* const bB = S_.bB;
+  Real* const bF = S_.bF;
+  Real* const sCoef = S_.sCoef;
+  uint32_t* const sOps = S_.sOps;
+  uint32_t* const sRO = S_.sRO;
+  int32_t* const sEi = S_.sEi;
+  uint8_t* const sFlag = S_.sFlag;
   bool first = true;
   for (;;) {
     uint32_t f;
     if (fam_single >= 0) {
-      if (!first) break;
+      if (!first || wv != 0) break;
       f = (uint32_t)fam_single;
     } else {
-      if (lane == 0) s_fam = fam_begin + atomicAdd(W.ticket, 1u);
-      __syncthreads();
-      uint32_t k = s_fam;
-      __syncthreads();
+      uint32_t k = 0;
+      if (lane == 0) k = atomicAdd(W.ticket, 1u);
+      k = fam_begin + __builtin_amdgcn_readfirstlane(k);
       if (k >= fam_end) break;
       f = B.order[k];
     }
     first = false;
     const FamDev F = B.fam[f];
     Real* ell = W.ell + F.ell0;
-    int32_t* E = W.E;
-    // scaled forms: coefficients of the stage's records, into LDS and (for the reverse sweep) global
-    auto refresh_coefs = [&](const StageDev& S, uint32_t rb) {
-      for (uint32_t i = lane; i < S.n_items; i += kWG) {
-        const uint32_t ia = S.item0 + i;
-        const int Eg = E[F.clade0 + B.it_gid[ia]];
-        for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) {
-          const Real c = scaled_coef<Real>(B, F, E, S.item0, k, Eg);
-          sCoef[k - rb] = c;
-          W.coef[k] = c;
-        }
-      }
-    };
-    if (Scaled) {
-      for (uint32_t c = lane; c < F.n_clades; c += kWG) E[F.clade0 + c] = kUndef;
-      __syncthreads();
-    }
+    int32_t* E = W.E + F.clade0;
+    if (Scaled)
+      for (uint32_t c = lane; c < F.n_clades; c += kWG) E[c] = kUndef;
     // ---------------- non-root stages
     for (uint32_t s = 0; s < F.n_stages; ++s) {
       const StageDev S = B.stage[F.stage0 + s];
-      const uint32_t n = S.n_items;
-      const uint32_t rb = B.it_rec[S.item0];             // the stage's records are [rb, rb + nrec)
-      const uint32_t nrec = B.it_rec[S.item0 + n] - rb;  // <= rcap (size class)
+      const uint32_t n = S.n_items, i0 = S.item0;
+      const uint32_t rb = B.it_rec[i0];
+      const uint32_t nrec = B.it_rec[i0 + n] - rb;  // <= rcap (size class)
+      wave_sync_mem();  // ℓ and exponents of the stages below
       for (uint32_t i = lane; i < n; i += kWG) {
-        int32_t cf = B.it_cf[S.item0 + i], cg = B.it_cg[S.item0 + i];
+        const uint32_t ia = i0 + i;
+        const int32_t cf = B.it_cf[ia], cg = B.it_cg[ia];
         bF[i] = cf >= 0 ? ell[cf] : Real(0);
-        bG[i] = cg >= 0 ? ell[cg] : Real(0);
+        bB[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops (bB is free until row 1)
+        sRO[i] = B.it_rec[ia] - rb;
+        if (Scaled) {
+          const int e = E[B.it_gid[ia]];
+          sEi[i] = e;
+          sFlag[i] = e == kUndef ? (kPending | kYoung) : 0;
+        }
       }
+      if (lane == 0) sRO[n] = nrec;
       for (uint32_t k = lane; k < nrec; k += kWG) {
         sOps[k] = B.rec_ops[rb + k];
-        if (!Scaled) sCoef[k] = coefs[rb + k];
+        sCoef[k] = (Real)B.rec_p[rb + k];  // p itself; the scaled forms rescale it after row 0
       }
-      __syncthreads();
-      // row 1
+      wave_sync();
+      // row 0: terms from the child branches (whale!/whalewgd!, src/core.jl:95-128)
       for (uint32_t i = lane; i < n; i += kWG) {
-        const uint32_t ia = S.item0 + i;
+        const uint32_t ia = i0 + i;
         const SegDev sg = B.seg[B.it_seg[ia]];
         const NodeDev& nd = M.node[sg.node];
-        const uint32_t gid = B.it_gid[ia];
-        const bool leafc = B.clade_size[F.clade0 + gid] == 1;
-        const uint32_t r0 = B.it_rec[ia], r1 = B.it_rec[ia + 1];
+        const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
         Real v = 0;
         if (nd.kind == WHALE_NODE_LEAF) {
-          if (leafc) {
+          if (B.clade_size[F.clade0 + B.it_gid[ia]] == 1) {
             if (Scaled) {
-              int Eg = E[F.clade0 + gid];
+              const int Eg = sEi[i];
               if (Eg == kUndef) {
                 int ex;
                 v = frexp_r((Real)nd.leafP, &ex);
-                E[F.clade0 + gid] = ex;
+                sEi[i] = ex;
+                sFlag[i] = kYoung;
               } else {
                 v = ldexp_r((Real)nd.leafP, -Eg);
               }
@@ -360,38 +421,38 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
             const Real qq = (Real)M.par[3 * M.n + nd.wgd];
             const Real ef = (Real)eps_top(M, nd.child0);
             Real r = 0;
-            for (uint32_t k = r0 - rb; k < r1 - rb; ++k) {
-              uint32_t ops = sOps[k];
+            for (uint32_t k = r0; k < r1; ++k) {
+              const uint32_t ops = sOps[k];
               r += sCoef[k] * bF[ops & 0xffff] * bF[ops >> 16];
             }
             v = r * qq + ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]);
           } else {
             const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
             Real sp = 0;
-            for (uint32_t k = r0 - rb; k < r1 - rb; ++k) {
-              uint32_t ops = sOps[k];
-              uint32_t a = ops & 0xffff, b = ops >> 16;
-              sp += sCoef[k] * (bF[a] * bG[b] + bG[a] * bF[b]);
+            for (uint32_t k = r0; k < r1; ++k) {
+              const uint32_t ops = sOps[k];
+              const uint32_t a = ops & 0xffff, b = ops >> 16;
+              sp += sCoef[k] * (bF[a] * bB[b] + bB[a] * bF[b]);
             }
-            v = sp + (bF[i] * eg + bG[i] * ef);
+            v = sp + (bF[i] * eg + bB[i] * ef);
           }
         } else {
-          // scaled: explicit exponents (operands come from other segments)
+          // scaled: explicit exponents (the operands come from other segments)
           const bool wgd = nd.kind == WHALE_NODE_WGD;
           const Real qq = wgd ? (Real)M.par[3 * M.n + nd.wgd] : Real(1);
           const Real ef = (Real)eps_top(M, nd.child0);
           const Real eg = wgd ? Real(0) : (Real)eps_top(M, nd.child1);
-          int Eg = E[F.clade0 + gid];
+          const int Eg = sEi[i];
           XSum<Real> xs;
-          Real base = 0;  // terms already at scale Eg
           for (uint32_t k = r0; k < r1; ++k) {
-            uint32_t ops = B.rec_ops[k];
-            uint32_t a = ops & 0xffff, b = ops >> 16;
-            Real pk = (Real)B.split_p[F.split0 + B.rec_sid[k]];
-            Real fa = bF[a], fb = bF[b];
-            Real ga = wgd ? fa : bG[a], gb = wgd ? fb : bG[b];
-            int ea = E[F.clade0 + B.it_gid[S.item0 + a]], eb = E[F.clade0 + B.it_gid[S.item0 + b]];
-            Real m; int ex;
+            const uint32_t ops = sOps[k];
+            const uint32_t a = ops & 0xffff, b = ops >> 16;
+            const Real pk = sCoef[k];
+            const Real fa = bF[a], fb = bF[b];
+            const Real ga = wgd ? fa : bB[a], gb = wgd ? fb : bB[b];
+            const int ea = sEi[a], eb = sEi[b];
+            Real m;
+            int ex;
             if (fa != Real(0) && gb != Real(0)) { xprod(pk * qq, fa, ea, gb, eb, &m, &ex); xs.add(m, ex); }
             if (!wgd && ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ex); xs.add(m, ex); }
           }
@@ -399,64 +460,98 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
             if (xs.k != kUndef) {
               int ex;
               v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
-              E[F.clade0 + gid] = xs.k + ex + kBirth;
+              sEi[i] = xs.k + ex + kBirth;
+              sFlag[i] = kYoung;
             }
           } else {
-            if (wgd) base = ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]);
-            else base = bF[i] * eg + bG[i] * ef;
+            const Real base = wgd ? ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]) : (bF[i] * eg + bB[i] * ef);
             v = base + (xs.k == kUndef ? Real(0) : ldexp_r(xs.s, xs.k - Eg));
           }
         }
         bA[i] = v;
-        const uint32_t j = i - sg.item0;
-        if (StoreAll || sg.R == 1) ell[sg.ell + j] = v;
+        if (StoreAll || sg.R == 1) ell[sg.ell + (i - sg.item0)] = v;
       }
-      __syncthreads();
+      wave_sync();
+      // scaled forms: coefficients for the exponents now fixed.  A record whose
+      // clade is still open keeps p (the open clade sums explicit exponents).
+      auto refresh = [&]() {
+        for (uint32_t i = lane; i < n; i += kWG) {
+          const int Eg = sEi[i];
+          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+            const uint32_t ops = sOps[k];
+            const Real p = (Real)B.rec_p[rb + k];
+            const Real c = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
+            sCoef[k] = Eg == kUndef ? p : c;
+            if (StoreAll) W.coef[rb + k] = c;
+          }
+        }
+      };
       if (Scaled) {
-        refresh_coefs(S, rb);
-        __syncthreads();
+        refresh();
+        wave_sync();
       }
-      // rows 2..R
+      // rows 1..R-1: within_branch! (src/core.jl:83-92)
       Real* cur = bA;
       Real* nxt = bB;
       ItemRow cache[kCache];
 #pragma unroll
       for (int c = 0; c < kCache; ++c) {
         const uint32_t i = lane + c * kWG;
-        if (i < n) cache[c] = load_item(B, M, S.item0 + i, i);
+        if (i < n) cache[c] = load_item(B, M, i0 + i, i, sRO);
       }
       for (uint32_t r = 1; r < S.r_max; ++r) {
-        if (Scaled && lane == 0) s_dirty = 0;  // one wave per workgroup: ordered after last row's read
+        bool dirty = false;
         auto update = [&](const ItemRow& it, uint32_t i) {
           Real v = cur[i];
           if (r < it.R) {
-            const Real ph = (Real)M.phi[it.tab + r], ps = (Real)M.psi[it.tab + r];
-            bool pending = false;
-            if (Scaled && v == Real(0) && it.r1 > it.r0) pending = E[F.clade0 + it.gid] == kUndef;
-            if (!pending) {
+            Real ph, ps;
+            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
+            const uint8_t fl = Scaled ? sFlag[i] : 0;
+            const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
+            if (!Scaled || !(fl & kPending) || r1 == r0) {
               Real acc = 0;
-              for (uint32_t k = it.r0 - rb; k < it.r1 - rb; ++k) {
-                uint32_t ops = sOps[k];
+              for (uint32_t k = r0; k < r1; ++k) {
+                const uint32_t ops = sOps[k];
                 acc += sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
               }
               v = ph * v + ps * acc;
+              if (Scaled && sizeof(Real) == sizeof(float) && (fl & kYoung) &&
+                  fabsf((float)v) > (float)(1ull << kBump) && isfinite((float)v)) {
+                // renormalise a young clade in place (its only values are this branch's rows);
+                // its exponent is raised after the row, as other items of this row still read
+                // the clade's previous row at the old scale
+                int ex;
+                (void)frexp_r(v, &ex);
+                const int sh = ex + kBirth;
+                v = ldexp_r(v, -sh);
+                if (StoreAll)
+                  for (uint32_t q = 0; q < r; ++q) {
+                    Real* p = ell + it.cell + (size_t)q * it.stride;
+                    *p = ldexp_r(*p, -sh);
+                  }
+                bF[i] = (Real)sh;  // bF is free after row 0
+                sFlag[i] = fl | kShifted;
+                dirty = true;
+              }
             } else {
+              // clade still open: sum with explicit exponents (coefficients hold p)
               XSum<Real> xs;
-              for (uint32_t k = it.r0; k < it.r1; ++k) {
-                uint32_t ops = sOps[k - rb];
-                uint32_t a = ops & 0xffff, b = ops >> 16;
-                Real va = cur[a], vb = cur[b];
+              for (uint32_t k = r0; k < r1; ++k) {
+                const uint32_t ops = sOps[k];
+                const uint32_t a = ops & 0xffff, b = ops >> 16;
+                const Real va = cur[a], vb = cur[b];
                 if (va == Real(0) || vb == Real(0)) continue;
-                int ea = E[F.clade0 + B.it_gid[S.item0 + a]], eb = E[F.clade0 + B.it_gid[S.item0 + b]];
-                Real m; int ex;
-                xprod((Real)B.split_p[F.split0 + B.rec_sid[k]], va, ea, vb, eb, &m, &ex);
+                Real m;
+                int ex;
+                xprod(sCoef[k], va, sEi[a], vb, sEi[b], &m, &ex);
                 xs.add(m, ex);
               }
               if (xs.k != kUndef) {
                 int ex;
                 v = ldexp_r(frexp_r(ps * xs.s, &ex), -kBirth);
-                E[F.clade0 + it.gid] = xs.k + ex + kBirth;
-                s_dirty = 1;
+                sEi[i] = xs.k + ex + kBirth;  // readers of row r-1 see 0 here and skip it
+                sFlag[i] = kYoung;
+                dirty = true;
               }
             }
             if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
@@ -468,20 +563,30 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
           const uint32_t i = lane + c * kWG;
           if (i < n) update(cache[c], i);
         }
-        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) update(load_item(B, M, S.item0 + i, i), i);
+        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) update(load_item(B, M, i0 + i, i, sRO), i);
         wave_sync();
-        if (Scaled && s_dirty) {
-          // an exponent was fixed in this row (E is global: full barrier)
-          __syncthreads();
-          refresh_coefs(S, rb);
-          __syncthreads();
+        if (Scaled && __any(dirty)) {
+          for (uint32_t i = lane; i < n; i += kWG) {
+            if (sFlag[i] & kShifted) {
+              sEi[i] += (int)bF[i];
+              sFlag[i] &= (uint8_t)~kShifted;
+            }
+          }
+          wave_sync();
+          refresh();
+          wave_sync();
         }
-        Real* t = cur; cur = nxt; nxt = t;
+        Real* t = cur;
+        cur = nxt;
+        nxt = t;
       }
-      __syncthreads();  // this stage's ℓ is read by later stages
+      // exponents of this stage's clades, for the stages above and ℓ download
+      if (Scaled)
+        for (uint32_t i = lane; i < n; i += kWG) E[B.it_gid[i0 + i]] = sEi[i];
     }
-    // ---------------- root: all clades, by size level
+    // ---------------- root: all clades, by size level (whaleroot!, src/core.jl:139-158)
     {
+      wave_sync_mem();
       const uint32_t n = F.n_clades;
       const uint32_t i0 = F.root_item0;
       const SegDev sg = B.seg[F.root_seg];
@@ -492,64 +597,78 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
       const Real cA = (Real)((1.0 - eta) * xi / eta);
       const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
       const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
+      const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
+      const bool inl = nrec <= rcap;  // records staged in LDS
       for (uint32_t i = lane; i < n; i += kWG) {
-        int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
+        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
         bF[i] = cf >= 0 ? ell[cf] : Real(0);
-        bG[i] = cg >= 0 ? ell[cg] : Real(0);
+        bB[i] = cg >= 0 ? ell[cg] : Real(0);
+        sRO[i] = B.it_rec[i0 + i] - rb;
+        if (Scaled) sEi[i] = E[i];  // root items are the clades in order
       }
-      __syncthreads();
-      for (uint32_t L = 0; L < F.n_lvl; ++L) {
-        const uint32_t a0 = B.lvl[F.lvl0 + L], a1 = B.lvl[F.lvl0 + L + 1];
+      if (lane == 0) sRO[n] = nrec;
+      if (inl)
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          sOps[k] = B.rec_ops[rb + k];
+          sCoef[k] = (Real)B.rec_p[rb + k];
+        }
+      wave_sync();
+      for (uint32_t Lv = 0; Lv < F.n_lvl; ++Lv) {
+        const uint32_t a0 = B.lvl[F.lvl0 + Lv], a1 = B.lvl[F.lvl0 + Lv + 1];
         for (uint32_t i = a0 + lane; i < a1; i += kWG) {
-          const uint32_t ia = i0 + i;
-          const uint32_t r0 = B.it_rec[ia], r1 = B.it_rec[ia + 1];
+          const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
           Real v;
           if (!Scaled) {
             Real a = 0, b = 0;
             for (uint32_t k = r0; k < r1; ++k) {
-              uint32_t ops = B.rec_ops[k];
-              uint32_t x = ops & 0xffff, y = ops >> 16;
-              a += coefs[k] * bA[x] * bA[y];
-              b += coefs[k] * (bF[x] * bG[y] + bG[x] * bF[y]);
+              const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
+              const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
+              const uint32_t x = ops & 0xffff, y = ops >> 16;
+              a += p * bA[x] * bA[y];
+              b += p * (bF[x] * bB[y] + bB[x] * bF[y]);
             }
-            Real c = bF[i] * eg + bG[i] * ef;
+            const Real c = bF[i] * eg + bB[i] * ef;
             v = cA * a + cB * (b + c);
           } else {
-            int Eg = E[F.clade0 + i];
+            const int Eg = sEi[i];
             XSum<Real> xs;
             for (uint32_t k = r0; k < r1; ++k) {
-              uint32_t ops = B.rec_ops[k];
-              uint32_t x = ops & 0xffff, y = ops >> 16;
-              Real pk = (Real)B.split_p[F.split0 + B.rec_sid[k]];
-              int ex_ = E[F.clade0 + x], ey = E[F.clade0 + y];
-              Real m; int ee;
+              const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
+              const Real pk = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
+              const uint32_t x = ops & 0xffff, y = ops >> 16;
+              const int ex_ = sEi[x], ey = sEi[y];
+              Real m;
+              int ee;
               if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); xs.add(m, ee); }
-              if (bF[x] != Real(0) && bG[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bG[y], ey, &m, &ee); xs.add(m, ee); }
-              if (bG[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bG[x], ex_, bF[y], ey, &m, &ee); xs.add(m, ee); }
+              if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); xs.add(m, ee); }
+              if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); xs.add(m, ee); }
             }
             if (Eg == kUndef) {
               v = 0;
               if (xs.k != kUndef) {
                 int ex;
                 v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
-                E[F.clade0 + i] = xs.k + ex + kBirth;
+                sEi[i] = xs.k + ex + kBirth;  // no clade of this level is a part of another
               }
             } else {
-              Real c = bF[i] * eg + bG[i] * ef;
+              const Real c = bF[i] * eg + bB[i] * ef;
               v = cB * c + (xs.k == kUndef ? Real(0) : ldexp_r(xs.s, xs.k - Eg));
             }
           }
           bA[i] = v;
-          ell[sg.ell + i] = v;
         }
-        __syncthreads();
+        wave_sync();
+      }
+      for (uint32_t i = lane; i < n; i += kWG) {
+        ell[sg.ell + i] = bA[i];
+        if (Scaled) E[i] = sEi[i];
       }
       if (lane == 0) {
-        Real L = bA[n - 1];
+        const Real Lr = bA[n - 1];
         double ll;
-        if (L > Real(0) && isfinite((double)L)) {
-          ll = log((double)L);
-          if (Scaled) ll += (double)E[F.clade0 + n - 1] * 0.69314718055994530942;
+        if (Lr > Real(0) && isfinite((double)Lr)) {
+          ll = log((double)Lr);
+          if (Scaled) ll += (double)sEi[n - 1] * 0.69314718055994530942;
         } else {
           ll = -INFINITY;
           if (Scaled && sizeof(Real) == sizeof(float)) {
@@ -563,12 +682,13 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
       if (Scaled && StoreAll) {
         // final coefficients of the root records, for the reverse sweep
         for (uint32_t i = lane; i < n; i += kWG) {
-          const uint32_t ia = i0 + i;
-          int Eg = E[F.clade0 + i];
-          for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) W.coef[k] = scaled_coef<Real>(B, F, E, i0, k, Eg);
+          const int Eg = sEi[i];
+          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+            const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
+            W.coef[rb + k] = coef_of((Real)B.rec_p[rb + k], sEi[ops & 0xffff], sEi[ops >> 16], Eg);
+          }
         }
       }
-      __syncthreads();
     }
   }
 }
@@ -576,34 +696,85 @@ __global__ __launch_bounds__(kWG) void k_forward(BatchDev B, ModelDev M, WorkDev
 // ---------------------------------------------------------------------------
 // backward (reverse sweep); requires a forward with StoreAll
 // ---------------------------------------------------------------------------
-template <typename Real, bool Scaled>
-__global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, uint32_t cap,
-                                                   uint32_t rcap, int32_t fam_single, uint32_t fam_begin,
-                                                   uint32_t fam_end, const uint32_t* fam_count) {
+// LDS of one wave: bA bB bV bG [cap] sCoef [rcap] (Real) | sOps [rcap] | sRO [cap+1].
+// Shared (TabL): φ, ψ per table row (Real), then the gradient accumulators in
+// the global layout (doubles), flushed to global memory once per workgroup.
+template <typename Real>
+struct BwdLds {
+  Real *bA, *bB, *bV, *bG, *sCoef;
+  uint32_t *sOps, *sRO;
+  __device__ BwdLds(unsigned char* wb, uint32_t cap, uint32_t rcap) {
+    bA = reinterpret_cast<Real*>(wb);
+    bB = bA + cap;
+    bV = bB + cap;
+    bG = bV + cap;
+    sCoef = bG + cap;
+    sOps = reinterpret_cast<uint32_t*>(wb + align16((4 * (size_t)cap + rcap) * sizeof(Real)));
+    sRO = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(sOps) + align16(4 * (size_t)rcap));
+  }
+};
+
+// per-item data of the row-0 step of the reverse sweep
+struct ItemTop {
+  uint32_t ib;      // ī cell (relative to ibar0) of this clade in the child0 slot; child1: + nc
+  uint32_t nc;
+  int32_t kind;
+  int32_t c0, c1;   // children
+  int32_t wgd;      // q index of a WGD node
+};
+
+__device__ __forceinline__ ItemTop load_top(const BatchDev& B, const ModelDev& M, uint32_t ia, uint32_t i) {
+  ItemTop t;
+  const SegDev sg = B.seg[B.it_seg[ia]];
+  const NodeDev& nd = M.node[sg.node];
+  t.ib = sg.ibar == kNone ? kNone : sg.ibar + (i - sg.item0);
+  t.nc = sg.nc;
+  t.kind = nd.kind;
+  t.c0 = nd.child0;
+  t.c1 = nd.child1;
+  t.wgd = nd.wgd;
+  return t;
+}
+
+template <typename Real, bool Scaled, int FPW, bool TabL>
+__global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
+                                                        const uint32_t* fam_count) {
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
-  Real* bA = reinterpret_cast<Real*>(smem);  // d logL / d ℓ, current row
-  Real* bB = bA + cap;                        // next row
-  Real* bV = bB + cap;                        // ℓ of the row below / child0 tops
-  Real* bG = bV + cap;                        // child1 tops
-  Real* sCoef = bG + cap;                                      // the stage's record coefficients
-  uint32_t* sOps = reinterpret_cast<uint32_t*>(sCoef + rcap);  // and operand pairs
-  __shared__ uint32_t s_fam;
-  const int lane = threadIdx.x;
-  const Real* coefs = Scaled ? W.coef : reinterpret_cast<const Real*>(B.rec_p);
+  const uint32_t lane = threadIdx.x % kWG;
+  const uint32_t wv = threadIdx.x / kWG;
   const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
-  double* acc = W.acc;
+  Real* sTab = reinterpret_cast<Real*>(smem);
+  double* const acc = TabL ? reinterpret_cast<double*>(smem + align16(2 * (size_t)M.tab_rows * sizeof(Real))) : W.acc;
+  if (TabL) {
+    for (int r = threadIdx.x; r < M.tab_rows; r += kWG * FPW) {
+      sTab[2 * r] = (Real)M.phi[r];
+      sTab[2 * r + 1] = (Real)M.psi[r];
+    }
+    for (int j = threadIdx.x; j < AL.n; j += kWG * FPW) acc[j] = 0.0;
+    __syncthreads();
+  }
+  const uint32_t cap = L.cap, rcap = L.rcap;
+  BwdLds<Real> S_(smem + L.shared_bytes + (size_t)wv * L.wave_bytes, cap, rcap);
+  Real* bA = S_.bA;  // d logL / d ℓ, current row
+  Real* bB = S_.bB;  // next row (and ī sums at row 0)
+  Real* const bV = S_.bV;  // ℓ of the row below / child0 tops
+  Real* const bG = S_.bG;  // child1 tops
+  Real* const sCoef = S_.sCoef;
+  uint32_t* const sOps = S_.sOps;
+  uint32_t* const sRO = S_.sRO;
+  const Real* coefs = Scaled ? W.coef : nullptr;
   bool first = true;
   for (;;) {
     uint32_t f;
     if (fam_single >= 0) {
-      if (!first) break;
+      if (!first || wv != 0) break;
       f = (uint32_t)fam_single;
     } else {
-      if (lane == 0) s_fam = fam_begin + atomicAdd(W.ticket, 1u);
-      __syncthreads();
-      uint32_t k = s_fam;
-      __syncthreads();
+      uint32_t k = 0;
+      if (lane == 0) k = atomicAdd(W.ticket, 1u);
+      k = fam_begin + __builtin_amdgcn_readfirstlane(k);
       if (k >= fam_end) break;
       f = B.order[k];
     }
@@ -612,6 +783,7 @@ __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDe
     const FamDev F = B.fam[f];
     const Real* ell = W.ell + F.ell0;
     Real* ibar = W.ibar + F.ibar0;
+    auto coef_at = [&](uint32_t k) -> Real { return Scaled ? coefs[k] : (Real)B.rec_p[k]; };
     // ---------------- root
     {
       const uint32_t n = F.n_clades;
@@ -624,54 +796,64 @@ __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDe
       const Real cA = (Real)((1.0 - eta) * xi / eta);
       const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
       const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
+      const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
+      const bool inl = nrec <= rcap;
       for (uint32_t i = lane; i < n; i += kWG) {
-        int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
+        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
         bV[i] = ell[sg.ell + i];
         bB[i] = cf >= 0 ? ell[cf] : Real(0);  // child0 tops
-        bG[i] = cg >= 0 ? ell[cg] : Real(0);
+        bG[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops
         bA[i] = 0;
+        sRO[i] = B.it_rec[i0 + i] - rb;
       }
-      __syncthreads();
+      if (lane == 0) sRO[n] = nrec;
+      if (inl)
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          sOps[k] = B.rec_ops[rb + k];
+          sCoef[k] = coef_at(rb + k);
+        }
+      wave_sync();
+      auto rec = [&](uint32_t k, uint32_t& ops, Real& c) {
+        if (inl) { ops = sOps[k]; c = sCoef[k]; }
+        else { ops = B.rec_ops[rb + k]; c = coef_at(rb + k); }
+      };
       const Real Lr = bV[n - 1];
       const bool ok = Lr > Real(0) && isfinite((double)Lr);
       if (ok) {
-        Real sA = 0, sBc = 0, sF = 0, sG = 0;
-        for (int L = (int)F.n_lvl - 1; L >= 0; --L) {
-          const uint32_t a0 = B.lvl[F.lvl0 + L], a1 = B.lvl[F.lvl0 + L + 1];
+        if (lane == 0) bA[n - 1] = Real(1) / Lr;
+        wave_sync();
+        // d logL / d ℓ of each root clade, larger clades first, scattered into the parts
+        for (int Lv = (int)F.n_lvl - 1; Lv >= 0; --Lv) {
+          const uint32_t a0 = B.lvl[F.lvl0 + Lv], a1 = B.lvl[F.lvl0 + Lv + 1];
           for (uint32_t i = a0 + lane; i < a1; i += kWG) {
-            const uint32_t ia = i0 + i;
-            Real g = (i == n - 1) ? Real(1) / Lr : Real(0);
-            for (uint32_t k = B.it_inv[ia]; k < B.it_inv[ia + 1]; ++k) {
-              uint32_t ops = B.inv_ops[k];
-              g += cA * coefs[B.inv_rec[k]] * bA[ops & 0xffff] * bV[ops >> 16];
+            const Real t = cA * bA[i];
+            if (t == Real(0)) continue;
+            for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+              uint32_t ops;
+              Real c;
+              rec(k, ops, c);
+              const uint32_t x = ops & 0xffff, y = ops >> 16;
+              lds_add(&bA[x], t * c * bV[y]);
+              lds_add(&bA[y], t * c * bV[x]);
             }
-            bA[i] = g;
           }
-          __syncthreads();
+          wave_sync();
         }
+        Real sA = 0, sBc = 0, sF = 0, sG = 0;
         for (uint32_t i = lane; i < n; i += kWG) {
-          const uint32_t ia = i0 + i;
           const Real g = bA[i];
-          Real sf = 0, sgg = 0;
-          for (uint32_t k = B.it_inv[ia]; k < B.it_inv[ia + 1]; ++k) {
-            uint32_t ops = B.inv_ops[k];
-            Real c = coefs[B.inv_rec[k]] * bA[ops & 0xffff];
-            uint32_t sib = ops >> 16;
-            sf += c * bG[sib];
-            sgg += c * bB[sib];
-          }
-          ibar[sg.ibar + i] = cB * (sf + g * eg);
-          ibar[sg.ibar + n + i] = cB * (sgg + g * ef);
           Real a = 0, b = 0;
-          for (uint32_t k = B.it_rec[ia]; k < B.it_rec[ia + 1]; ++k) {
-            uint32_t ops = B.rec_ops[k];
-            uint32_t x = ops & 0xffff, y = ops >> 16;
-            a += coefs[k] * bV[x] * bV[y];
-            b += coefs[k] * (bB[x] * bG[y] + bG[x] * bB[y]);
+          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+            uint32_t ops;
+            Real c;
+            rec(k, ops, c);
+            const uint32_t x = ops & 0xffff, y = ops >> 16;
+            a += c * bV[x] * bV[y];
+            b += c * (bB[x] * bG[y] + bG[x] * bB[y]);
           }
-          Real c = bB[i] * eg + bG[i] * ef;
+          const Real cc = bB[i] * eg + bG[i] * ef;
           sA += g * a;
-          sBc += g * (b + c);
+          sBc += g * (b + cc);
           sF += g * bB[i];
           sG += g * bG[i];
         }
@@ -680,155 +862,242 @@ __global__ __launch_bounds__(kWG) void k_backward(BatchDev B, ModelDev M, WorkDe
         sF = wave_sum(sF);
         sG = wave_sum(sG);
         if (lane == 0) {
-          unsafeAtomicAdd(acc + AL.rootA, (double)sA);
-          unsafeAtomicAdd(acc + AL.rootB, (double)sBc);
-          unsafeAtomicAdd(acc + AL.eps + nd.child1, (double)cB * (double)sF);
-          unsafeAtomicAdd(acc + AL.eps + nd.child0, (double)cB * (double)sG);
+          acc_add<TabL>(acc + AL.rootA, (double)sA);
+          acc_add<TabL>(acc + AL.rootB, (double)sBc);
+          acc_add<TabL>(acc + AL.eps + nd.child1, (double)cB * (double)sF);
+          acc_add<TabL>(acc + AL.eps + nd.child0, (double)cB * (double)sG);
+        }
+        // ī of the child tops: child0 slot takes ℓ of child1 tops and vice versa
+        wave_sync();
+        for (int slot = 0; slot < 2; ++slot) {
+          const Real* other = slot == 0 ? bG : bB;
+          const Real eo = slot == 0 ? eg : ef;
+          for (uint32_t i = lane; i < n; i += kWG) bV[i] = bA[i] * eo;
+          wave_sync();
+          for (uint32_t i = lane; i < n; i += kWG) {
+            const Real g = bA[i];
+            if (g == Real(0)) continue;
+            for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+              uint32_t ops;
+              Real c;
+              rec(k, ops, c);
+              const uint32_t x = ops & 0xffff, y = ops >> 16;
+              lds_add(&bV[x], g * c * other[y]);
+              lds_add(&bV[y], g * c * other[x]);
+            }
+          }
+          wave_sync();
+          for (uint32_t i = lane; i < n; i += kWG) ibar[sg.ibar + slot * n + i] = cB * bV[i];
+          wave_sync();
         }
       } else {
         for (uint32_t i = lane; i < 2 * n; i += kWG) ibar[sg.ibar + i] = 0;
       }
-      __syncthreads();
     }
     // ---------------- non-root stages, top down
     for (int s = (int)F.n_stages - 1; s >= 0; --s) {
       const StageDev S = B.stage[F.stage0 + s];
-      const uint32_t n = S.n_items;
-      const uint32_t rb = B.it_rec[S.item0];
-      const uint32_t nrec = B.it_rec[S.item0 + n] - rb;  // <= rcap (size class)
-      for (uint32_t i = lane; i < n; i += kWG) bA[i] = ibar[B.it_bar[S.item0 + i]];
+      const uint32_t n = S.n_items, i0 = S.item0;
+      const uint32_t rb = B.it_rec[i0];
+      const uint32_t nrec = B.it_rec[i0 + n] - rb;  // <= rcap (size class)
+      wave_sync_mem();  // ī written by the stage above
+      for (uint32_t i = lane; i < n; i += kWG) {
+        bA[i] = ibar[B.it_bar[i0 + i]];
+        sRO[i] = B.it_rec[i0 + i] - rb;
+      }
+      if (lane == 0) sRO[n] = nrec;
       for (uint32_t k = lane; k < nrec; k += kWG) {
         sOps[k] = B.rec_ops[rb + k];
-        sCoef[k] = coefs[rb + k];
+        sCoef[k] = coef_at(rb + k);
       }
-      __syncthreads();
+      wave_sync();
       ItemRow cache[kCache];
+      Real pre[kCache];  // ℓ of the next row down, loaded one row ahead
+      const int rtop = (int)S.r_max - 1;
 #pragma unroll
       for (int c = 0; c < kCache; ++c) {
         const uint32_t i = lane + c * kWG;
-        if (i < n) cache[c] = load_item(B, M, S.item0 + i, i);
+        pre[c] = 0;
+        if (i < n) {
+          cache[c] = load_item(B, M, i0 + i, i, sRO);
+          if (rtop >= 1 && rtop < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(rtop - 1) * cache[c].stride];
+        }
       }
-      for (int r = (int)S.r_max - 1; r >= 1; --r) {
-        // ℓ of the row below, for the items whose segment reaches row r
+      for (int r = rtop; r >= 1; --r) {
+        // ℓ of row r-1 for the items whose segment reaches row r
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
           const uint32_t i = lane + c * kWG;
-          if (i < n && r < (int)cache[c].R) bV[i] = ell[cache[c].cell + (size_t)(r - 1) * cache[c].stride];
+          if (i < n && r < (int)cache[c].R) bV[i] = pre[c];
         }
         for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) {
-          const ItemRow it = load_item(B, M, S.item0 + i, i);
+          const ItemRow it = load_item(B, M, i0 + i, i, sRO);
           if (r < (int)it.R) bV[i] = ell[it.cell + (size_t)(r - 1) * it.stride];
         }
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n && r >= 2 && r - 1 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(r - 2) * cache[c].stride];
+        }
         wave_sync();
-        // all lanes call this together (segmented reductions)
-        auto update = [&](const ItemRow& it, uint32_t i, bool valid) {
+        // φ part of d logL / d ℓ(row r-1)
+        auto phi_part = [&](const ItemRow& it, uint32_t i) {
+          const Real g = bA[i];
+          if (r < (int)it.R) {
+            Real ph, ps;
+            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
+            bB[i] = ph * g;
+          } else {
+            bB[i] = g;
+          }
+        };
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n) phi_part(cache[c], i);
+        }
+        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) phi_part(load_item(B, M, i0 + i, i, sRO), i);
+        wave_sync();
+        // ψ part scattered into the parts of each split; d/dφ_r, d/dψ_r per segment
+        // (all lanes take part: segmented reductions)
+        auto psi_part = [&](const ItemRow& it, uint32_t i, bool valid) {
           Real cphi = 0, cpsi = 0;
           uint32_t key = kNone;
-          double* addr = acc;
-          if (valid) {
-            Real g = bA[i];
-            if (r < (int)it.R) {
-              const Real ph = (Real)M.phi[it.tab + r], ps = (Real)M.psi[it.tab + r];
-              Real sd = 0;
-              for (uint32_t k = it.q0; k < it.q1; ++k) {
-                uint32_t ops = B.inv_ops[k];
-                sd += sCoef[B.inv_rec[k] - rb] * bA[ops & 0xffff] * bV[ops >> 16];
+          if (valid && r < (int)it.R) {
+            const Real g = bA[i];
+            Real ph, ps;
+            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
+            const Real t = ps * g;
+            Real fo = 0;
+            const uint32_t r1 = sRO[i + 1];
+            for (uint32_t k = sRO[i]; k < r1; ++k) {
+              const uint32_t ops = sOps[k];
+              const uint32_t a = ops & 0xffff, b = ops >> 16;
+              const Real c = sCoef[k];
+              const Real va = bV[a], vb = bV[b];
+              fo += c * va * vb;
+              if (t != Real(0)) {
+                lds_add(&bB[a], t * c * vb);
+                lds_add(&bB[b], t * c * va);
               }
-              Real fo = 0;
-              for (uint32_t k = it.r0 - rb; k < it.r1 - rb; ++k) {
-                uint32_t ops = sOps[k];
-                fo += sCoef[k] * bV[ops & 0xffff] * bV[ops >> 16];
-              }
-              bB[i] = ph * g + ps * sd;
-              cphi = g * bV[i];
-              cpsi = g * fo;
-              key = it.seg;
-              addr = acc + it.tab + r;
-            } else {
-              bB[i] = g;
             }
+            cphi = g * bV[i];
+            cpsi = g * fo;
+            key = it.tab;
           }
-          seg_add(cphi, key, addr);
-          seg_add(cpsi, key, addr + AL.psi);
+          bool tail;
+          seg_scan2(cphi, cpsi, key, &tail);
+          if (tail && key != kNone) {
+            acc_add<TabL>(acc + AL.phi + key + r, (double)cphi);
+            acc_add<TabL>(acc + AL.psi + key + r, (double)cpsi);
+          }
         };
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
           if ((uint32_t)c * kWG < n) {
             const uint32_t i = lane + c * kWG;
-            update(cache[c], i, i < n);
+            psi_part(cache[c], i, i < n);
           }
         }
         for (uint32_t base = kCache * kWG; base < n; base += kWG) {
           const uint32_t i = base + lane;
-          update(i < n ? load_item(B, M, S.item0 + i, i) : cache[0], i, i < n);
+          psi_part(i < n ? load_item(B, M, i0 + i, i, sRO) : cache[0], i, i < n);
         }
         wave_sync();
-        Real* t = bA; bA = bB; bB = t;
+        Real* t = bA;
+        bA = bB;
+        bB = t;
       }
-      // row 1: adjoints of the child tops, ε and q
+      // row 0: adjoints of the child tops, ε and q
       for (uint32_t i = lane; i < n; i += kWG) {
-        int32_t cf = B.it_cf[S.item0 + i], cg = B.it_cg[S.item0 + i];
+        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
         bV[i] = cf >= 0 ? ell[cf] : Real(0);
         bG[i] = cg >= 0 ? ell[cg] : Real(0);
       }
-      __syncthreads();
-      for (uint32_t base = 0; base < n; base += kWG) {
-        const uint32_t i = base + lane;
-        Real c0 = 0, c1 = 0, c2 = 0;
-        uint32_t key = kNone;
-        double* a0 = acc;
-        double* a1 = acc;
-        double* a2 = acc;
-        if (i < n) {
-          const uint32_t ia = S.item0 + i;
-          const uint32_t si = B.it_seg[ia];
-          const SegDev sg = B.seg[si];
-          const NodeDev& nd = M.node[sg.node];
-          const Real g = bA[i];
-          const uint32_t j = i - sg.item0;
-          if (nd.kind == WHALE_NODE_SPECIATION) {
-            const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
-            Real sf = 0, sgg = 0;
-            for (uint32_t k = B.it_inv[ia]; k < B.it_inv[ia + 1]; ++k) {
-              uint32_t ops = B.inv_ops[k];
-              Real c = sCoef[B.inv_rec[k] - rb] * bA[ops & 0xffff];
-              uint32_t sib = ops >> 16;
-              sf += c * bG[sib];
-              sgg += c * bV[sib];
-            }
-            ibar[sg.ibar + j] = sf + g * eg;
-            ibar[sg.ibar + sg.nc + j] = sgg + g * ef;
-            c0 = g * bV[i];  // -> ε of child1's top
-            c1 = g * bG[i];  // -> ε of child0's top
-            key = si;
-            a0 = acc + AL.eps + nd.child1;
-            a1 = acc + AL.eps + nd.child0;
-          } else if (nd.kind == WHALE_NODE_WGD) {
-            const Real qq = (Real)M.par[3 * M.n + nd.wgd];
-            const Real ef = (Real)eps_top(M, nd.child0);
-            Real sf = 0;
-            for (uint32_t k = B.it_inv[ia]; k < B.it_inv[ia + 1]; ++k) {
-              uint32_t ops = B.inv_ops[k];
-              sf += sCoef[B.inv_rec[k] - rb] * bA[ops & 0xffff] * bV[ops >> 16];
-            }
-            ibar[sg.ibar + j] = qq * sf + g * ((Real(1) - qq) + Real(2) * qq * ef);
-            Real ret = 0;
-            for (uint32_t k = B.it_rec[ia] - rb; k < B.it_rec[ia + 1] - rb; ++k) {
-              uint32_t ops = sOps[k];
-              ret += sCoef[k] * bV[ops & 0xffff] * bV[ops >> 16];
-            }
-            c0 = g * (ret - bV[i] + Real(2) * ef * bV[i]);  // -> q
-            c1 = g * Real(2) * qq * bV[i];                   // -> ε of child0's top
-            key = si;
-            a0 = acc + AL.q + nd.wgd;
-            a1 = acc + AL.eps + nd.child0;
-          }
-        }
-        seg_add(c0, key, a0);
-        seg_add(c1, key, a1);
-        (void)c2; (void)a2;
+      ItemTop tops[kCache];
+#pragma unroll
+      for (int c = 0; c < kCache; ++c) {
+        const uint32_t i = lane + c * kWG;
+        if (i < n) tops[c] = load_top(B, M, i0 + i, i);
       }
-      __syncthreads();
+      wave_sync();
+      // pass over all items: cached tops for the first kCache per lane
+      auto each = [&](auto&& fn) {
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n) fn(tops[c], i);
+        }
+        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) fn(load_top(B, M, i0 + i, i), i);
+      };
+      // ε̄, q̄ and the direct terms of the child0 slot
+      each([&](const ItemTop& tp, uint32_t i) {
+        const Real g = bA[i];
+        if (tp.kind == WHALE_NODE_SPECIATION) {
+          const Real eg = (Real)eps_top(M, tp.c1);
+          acc_add<TabL>(acc + AL.eps + tp.c1, (double)(g * bV[i]));
+          acc_add<TabL>(acc + AL.eps + tp.c0, (double)(g * bG[i]));
+          bB[i] = g * eg;
+        } else if (tp.kind == WHALE_NODE_WGD) {
+          const Real qq = (Real)M.par[3 * M.n + tp.wgd];
+          const Real ef = (Real)eps_top(M, tp.c0);
+          Real ret = 0;
+          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+            const uint32_t ops = sOps[k];
+            ret += sCoef[k] * bV[ops & 0xffff] * bV[ops >> 16];
+          }
+          acc_add<TabL>(acc + AL.q + tp.wgd, (double)(g * (ret - bV[i] + Real(2) * ef * bV[i])));
+          acc_add<TabL>(acc + AL.eps + tp.c0, (double)(g * Real(2) * qq * bV[i]));
+          bB[i] = g * ((Real(1) - qq) + Real(2) * qq * ef);
+        }
+      });
+      wave_sync();
+      // split terms of the child0 slot: speciation pairs child0 parts with child1
+      // tops of the other part; a WGD pairs both parts on its one child
+      each([&](const ItemTop& tp, uint32_t i) {
+        if (tp.kind != WHALE_NODE_SPECIATION && tp.kind != WHALE_NODE_WGD) return;
+        const bool wgd = tp.kind == WHALE_NODE_WGD;
+        const Real g = wgd ? (Real)M.par[3 * M.n + tp.wgd] * bA[i] : bA[i];
+        if (g == Real(0)) return;
+        const Real* other = wgd ? bV : bG;
+        for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+          const uint32_t ops = sOps[k];
+          const uint32_t a = ops & 0xffff, b = ops >> 16;
+          const Real c = sCoef[k];
+          lds_add(&bB[a], g * c * other[b]);
+          lds_add(&bB[b], g * c * other[a]);
+        }
+      });
+      wave_sync();
+      each([&](const ItemTop& tp, uint32_t i) {
+        if (tp.kind == WHALE_NODE_SPECIATION || tp.kind == WHALE_NODE_WGD) ibar[tp.ib] = bB[i];
+        if (tp.kind == WHALE_NODE_SPECIATION) bB[i] = bA[i] * (Real)eps_top(M, tp.c0);
+      });
+      wave_sync();
+      // child1 slot of speciation nodes
+      each([&](const ItemTop& tp, uint32_t i) {
+        if (tp.kind != WHALE_NODE_SPECIATION) return;
+        const Real g = bA[i];
+        if (g == Real(0)) return;
+        for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
+          const uint32_t ops = sOps[k];
+          const uint32_t a = ops & 0xffff, b = ops >> 16;
+          const Real c = sCoef[k];
+          lds_add(&bB[a], g * c * bV[b]);
+          lds_add(&bB[b], g * c * bV[a]);
+        }
+      });
+      wave_sync();
+      each([&](const ItemTop& tp, uint32_t i) {
+        if (tp.kind == WHALE_NODE_SPECIATION) ibar[tp.ib + tp.nc] = bB[i];
+      });
+    }
+  }
+  if (TabL) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < AL.n; j += kWG * FPW) {
+      const double v = acc[j];
+      if (v != 0.0) unsafeAtomicAdd(W.acc + j, v);
     }
   }
 }
@@ -951,54 +1220,82 @@ __global__ void k_fill_f64(double* p, double v, size_t n) {
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
-size_t lds_bytes(int mode, uint32_t cap, uint32_t rcap) {
-  size_t r = mode == WHALE_FP32_SCALED ? sizeof(float) : sizeof(double);
-  return (4 * (size_t)cap + rcap) * r + (size_t)rcap * sizeof(uint32_t);
+static size_t real_size(int mode) { return mode == WHALE_FP32_SCALED ? sizeof(float) : sizeof(double); }
+
+size_t fwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap) {
+  size_t b = align16((3 * (size_t)cap + rcap) * real_size(mode)) + align16(4 * (size_t)rcap) +
+             align16(4 * ((size_t)cap + 1));
+  if (mode != WHALE_FP64_PLAIN) b += align16(4 * (size_t)cap) + align16(cap);
+  return b;
 }
 
-template <typename K>
-static hipError_t lds_attr(K kern, size_t lds) {
-  static size_t done = 0;
-  (void)done;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+size_t bwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap) {
+  return align16((4 * (size_t)cap + rcap) * real_size(mode)) + align16(4 * (size_t)rcap) +
+         align16(4 * ((size_t)cap + 1));
 }
 
-hipError_t launch_forward(int mode, bool store_all, int grid, uint32_t cap, uint32_t rcap, hipStream_t st,
-                          const BatchDev& B, const ModelDev& M, const WorkRaw& W, int32_t fam_single,
-                          uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count) {
-  size_t lds = lds_bytes(mode, cap, rcap);
-  dim3 g(grid), b(kWG);
-#define WHALE_FWD(R, S, A)                                                                              \
-  do {                                                                                                  \
-    auto k = k_forward<R, S, A>;                                                                        \
-    if (lds > 65536) { hipError_t e = lds_attr(k, lds); if (e != hipSuccess) return e; }               \
-    hipLaunchKernelGGL(k, g, b, lds, st, B, M, WorkDev<R>(W), cap, rcap, fam_single, fam_begin, fam_end,    \
-                       fam_count);                                                                 \
-  } while (0)
-  if (mode == WHALE_FP64_PLAIN) { if (store_all) WHALE_FWD(double, false, true); else WHALE_FWD(double, false, false); }
-  else if (mode == WHALE_FP32_SCALED) { if (store_all) WHALE_FWD(float, true, true); else WHALE_FWD(float, true, false); }
-  else { if (store_all) WHALE_FWD(double, true, true); else WHALE_FWD(double, true, false); }
-#undef WHALE_FWD
+size_t fwd_shared_bytes(int mode, int tab_rows) { return align16(2 * (size_t)tab_rows * real_size(mode)); }
+
+size_t bwd_shared_bytes(int mode, int tab_rows, int acc_n) {
+  return align16(2 * (size_t)tab_rows * real_size(mode)) + align16(8 * (size_t)acc_n);
+}
+
+template <typename K, typename... Args>
+static hipError_t go(K kern, int grid, int fpw, size_t lds, hipStream_t st, Args... args) {
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(kWG * fpw), lds, st, args...);
   return hipGetLastError();
 }
 
-hipError_t launch_backward(int mode, int grid, uint32_t cap, uint32_t rcap, hipStream_t st, const BatchDev& B,
-                           const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
-                           uint32_t fam_end, const uint32_t* fam_count) {
-  size_t lds = lds_bytes(mode, cap, rcap);
-  dim3 g(grid), b(kWG);
-#define WHALE_BWD(R, S)                                                                                 \
-  do {                                                                                                  \
-    auto k = k_backward<R, S>;                                                                          \
-    if (lds > 65536) { hipError_t e = lds_attr(k, lds); if (e != hipSuccess) return e; }               \
-    hipLaunchKernelGGL(k, g, b, lds, st, B, M, WorkDev<R>(W), cap, rcap, fam_single, fam_begin, fam_end,    \
-                       fam_count);                                                                 \
-  } while (0)
-  if (mode == WHALE_FP64_PLAIN) WHALE_BWD(double, false);
-  else if (mode == WHALE_FP32_SCALED) WHALE_BWD(float, true);
-  else WHALE_BWD(double, true);
-#undef WHALE_BWD
-  return hipGetLastError();
+template <typename R, bool S, bool A>
+static hipError_t fwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                              const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
+                              const uint32_t* fam_count) {
+  const size_t lds = C.lds();
+  const WorkDev<R> w(W);
+  if (C.tabl && C.fpw == 4)
+    return go(k_forward<R, S, A, 4, true>, C.grid, 4, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  if (C.tabl)
+    return go(k_forward<R, S, A, 1, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  return go(k_forward<R, S, A, 1, false>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+}
+
+template <typename R, bool S>
+static hipError_t bwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                              const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
+                              const uint32_t* fam_count) {
+  const size_t lds = C.lds();
+  const WorkDev<R> w(W);
+  if (C.tabl && C.fpw == 4)
+    return go(k_backward<R, S, 4, true>, C.grid, 4, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  if (C.tabl)
+    return go(k_backward<R, S, 1, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  return go(k_backward<R, S, 1, false>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+}
+
+hipError_t launch_forward(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
+                          const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
+                          uint32_t fam_end, const uint32_t* fam_count) {
+  if (mode == WHALE_FP64_PLAIN)
+    return store_all ? fwd_variant<double, false, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count)
+                     : fwd_variant<double, false, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+  if (mode == WHALE_FP32_SCALED)
+    return store_all ? fwd_variant<float, true, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count)
+                     : fwd_variant<float, true, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+  return store_all ? fwd_variant<double, true, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count)
+                   : fwd_variant<double, true, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+}
+
+hipError_t launch_backward(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                           const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
+                           const uint32_t* fam_count) {
+  if (mode == WHALE_FP64_PLAIN) return bwd_variant<double, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+  if (mode == WHALE_FP32_SCALED) return bwd_variant<float, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+  return bwd_variant<double, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
 }
 
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,

@@ -1,5 +1,5 @@
 // Execution plan of a batch of gene families, shared by the host compiler
-// (plan.cpp) and the device kernels (kernels.hip).
+// (host.cpp) and the device kernels (kernels.hip).
 //
 // The reference evaluates one family by walking the species-tree nodes in
 // wm.order and, per node, every compatible clade and every time slice
@@ -53,6 +53,7 @@ struct FamDev {
   uint32_t max_items;    // largest stage, root included
   uint32_t max_recs;     // most records of any non-root stage
   uint32_t root_seg;     // absolute segment of the root
+  uint32_t root_recs;    // records of the root stage (all splits of the family)
   uint64_t ell0;         // first ℓ cell (absolute)
   uint64_t ibar0;        // first ī cell (absolute)
 };
