@@ -126,7 +126,8 @@ def main():
     log(f"parsed and planned in {time.time() - t0:.1f}s: {st}")
     for r in batch.launches("fp32"):
         log("launch fp32 {kernel} class {class}: {families} families, {per_workgroup}/workgroup, tables in LDS "
-            "{tables_in_lds}, grid {grid}, LDS {lds_bytes} B (cap {cap}, rcap {rcap})".format(**r))
+            "{tables_in_lds}, grid {grid}, LDS {lds_bytes} B ({shared_lds_bytes} shared, {family_lds_bytes} per "
+            "family)".format(**r))
     n_fam = st["families"]
     ctx = w.context()
     h = model.s.handle()

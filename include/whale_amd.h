@@ -145,7 +145,7 @@ int whale_batch_fallbacks(const whale_batch* b, int32_t* n);
 /* Launch shapes of the batched kernels for `mode`: for the forward and the
  * reverse sweep of each of the three size classes (6 rows of 8 values):
  * [families, families per workgroup, slice tables in LDS (0/1), grid,
- *  LDS bytes per workgroup, items per stage (cap), records per stage (rcap),
+ *  LDS bytes per workgroup, of which shared by its waves, 0 (reserved),
  *  LDS bytes per family].  No reference counterpart (introspection). */
 int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* out48);
 

@@ -81,10 +81,12 @@ def test_graph_replays_match_eager(w, cfg1):
                 v, g, fam = w.gradient(m, c, mode=mode)
                 results.append((rep, timing, mode, v, g, fam))
     _lib.lib().whale_ctx_set_timing(ctx, 0)
-    for mode in ("fp64", "fp32"):
+    # gradient sums are atomic (order varies run to run); FP32 runs accumulate
+    # per-stage partials in FP32 before the FP64 sum, so they vary a little more
+    for mode, gtol in (("fp64", 1e-9), ("fp32", 1e-7)):
         rs = [r for r in results if r[2] == mode]
         base = rs[0]
         for r in rs[1:]:
             assert np.array_equal(r[5], base[5])  # per-family logL: no atomics, bit-identical
             assert abs(r[3] - base[3]) <= 1e-12 * abs(base[3])
-            assert np.max(np.abs(r[4] - base[4])) <= 1e-9 * np.max(np.abs(base[4]))
+            assert np.max(np.abs(r[4] - base[4])) <= gtol * np.max(np.abs(base[4]))

@@ -143,7 +143,8 @@ class DeviceBatch:
         """Launch shape of each batched kernel: forward and reverse sweep per size class."""
         o = np.zeros(48, np.int64)
         check(_lib.lib().whale_batch_launches(self.h, _lib.MODES[mode], ptr(o, i64p)), "whale_batch_launches")
-        keys = ["families", "per_workgroup", "tables_in_lds", "grid", "lds_bytes", "cap", "rcap", "family_lds_bytes"]
+        keys = ["families", "per_workgroup", "tables_in_lds", "grid", "lds_bytes", "shared_lds_bytes", "reserved",
+                "family_lds_bytes"]
         out = []
         for d, name in enumerate(("forward", "backward")):
             for k in range(3):

@@ -62,6 +62,18 @@ hipError_t upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
 struct whale_batch;
 namespace {
 int plan_batch_launches(whale_batch* b);
+
+// LDS one wave needs for family d: its largest non-root stage (items and
+// records bounded separately) and its root stage, root records staged or not
+size_t fam_lds(const whale::FamDev& d, int mode, bool fwd, bool root_recs) {
+  const size_t s = mode == WHALE_FP32_SCALED ? 4 : 8;
+  const bool scaled = mode != WHALE_FP64_PLAIN;
+  const size_t nr = root_recs ? d.root_recs : 0;
+  if (fwd)
+    return std::max(whale::fwd_stage_bytes(s, scaled, d.max_items, d.max_recs),
+                    whale::fwd_root_bytes(s, scaled, d.n_clades, nr));
+  return std::max(whale::bwd_stage_bytes(s, d.max_items, d.max_recs), whale::bwd_root_bytes(s, d.n_clades, nr));
+}
 }  // namespace
 
 struct whale_ctx {
@@ -108,7 +120,7 @@ struct whale_batch {
   whale_model* model = nullptr;
   int device = 0;
   PlanHost plan;
-  DBuf fam, stage, seg, it_seg, it_gid, it_rec, it_inv, it_cf, it_cg, it_bar, rec_ops, rec_sid, inv_ops, inv_rec,
+  DBuf fam, stage, seg, it_seg, it_gid, it_rec, it_inv, it_cf, it_cg, it_bar, rec_ops, rec_sid, rec_own, inv_ops, inv_rec,
       rec_p, clade_size, split_p, lvl, order;
   // workspace.  ell/coef/ibar hold the main pass (FP32 or FP64 values) in
   // their first 8 bytes per element and, for FP32 runs, the FP64 recomputation
@@ -116,11 +128,9 @@ struct whale_batch {
   DBuf ell, coef, E, ibar, fam_ll, acc, ticket, flag, fb;
   uint64_t n_rec = 0;
   uint32_t n_fam = 0;
-  // size classes: families order[cls_lo[c] .. cls_hi[c]) hold stages of at most
-  // cls_cap[c] items and cls_rcap[c] records (class 2: the batch maximum)
+  // size classes by the LDS one family needs: families order[cls_lo[c] .. cls_hi[c])
   static constexpr int kClasses = 3;
   uint32_t cls_lo[kClasses] = {}, cls_hi[kClasses] = {};
-  uint32_t cls_cap[kClasses] = {}, cls_rcap[kClasses] = {};
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   int last_mode = -1;
@@ -150,6 +160,7 @@ struct whale_batch {
     B.it_bar = it_bar.as<int32_t>();
     B.rec_ops = rec_ops.as<uint32_t>();
     B.rec_sid = rec_sid.as<uint32_t>();
+    B.rec_own = rec_own.as<uint16_t>();
     B.inv_ops = inv_ops.as<uint32_t>();
     B.inv_rec = inv_rec.as<uint32_t>();
     B.rec_p = rec_p.as<double>();
@@ -503,19 +514,16 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   if (P.rec_ops.size() > 0xffffffffull || P.inv_ops.size() > 0xffffffffull || P.it_seg.size() > 0xffffffffull)
     return fail(WHALE_EINVAL, "batch too large");
   b->n_fam = (uint32_t)n_fam;
-  // size classes: families whose stages (root included) fit a small LDS budget
-  // run many to a compute unit.  Order: large class first, then small, then
-  // tiny; within a class the costliest families first.
+  // size classes by the LDS one family needs (FP32, either direction): most
+  // families fit a small wave region and run eight to a compute unit.  Order:
+  // large class first, then small, then tiny; costliest families first.
   {
-    const uint32_t caps[2] = {256, 512}, rcaps[2] = {512, 1024};
+    const size_t bounds[2] = {8 * 1024, 16 * 1024};
     std::vector<uint32_t> cls[whale_batch::kClasses];
     for (uint32_t f : P.cost_order) {
-      const FamDev& d = P.fam[f];
-      const uint32_t recs = std::max(d.max_recs, d.root_recs);
-      int k = 2;
-      for (int j = 1; j >= 0; --j)
-        if (d.max_items <= caps[j] && recs <= rcaps[j]) k = j;
-      cls[k].push_back(f);
+      const size_t need = std::max(fam_lds(P.fam[f], WHALE_FP32_SCALED, true, true),
+                                   fam_lds(P.fam[f], WHALE_FP32_SCALED, false, true));
+      cls[need <= bounds[0] ? 0 : need <= bounds[1] ? 1 : 2].push_back(f);
     }
     b->order_host.clear();
     for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
@@ -523,12 +531,6 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
       b->order_host.insert(b->order_host.end(), cls[k].begin(), cls[k].end());
       b->cls_hi[k] = (uint32_t)b->order_host.size();
     }
-    for (int j = 0; j < 2; ++j) {
-      b->cls_cap[j] = caps[j];
-      b->cls_rcap[j] = rcaps[j];
-    }
-    b->cls_cap[2] = std::max(P.max_items, 16u);
-    b->cls_rcap[2] = std::max(std::max(P.max_recs, P.max_root_recs), 4u);
   }
   {
     const int rc = plan_batch_launches(b.get());
@@ -549,6 +551,7 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   HIPCHK(upload(b->it_bar, P.it_bar, st));
   HIPCHK(upload(b->rec_ops, P.rec_ops, st));
   HIPCHK(upload(b->rec_sid, P.rec_sid, st));
+  HIPCHK(upload(b->rec_own, P.rec_own, st));
   HIPCHK(upload(b->inv_ops, P.inv_ops, st));
   HIPCHK(upload(b->inv_rec, P.inv_rec, st));
   HIPCHK(upload(b->rec_p, P.rec_p, st));
@@ -577,6 +580,7 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   std::vector<int32_t>().swap(P.it_bar);
   std::vector<uint32_t>().swap(P.rec_ops);
   std::vector<uint32_t>().swap(P.rec_sid);
+  std::vector<uint16_t>().swap(P.rec_own);
   std::vector<uint32_t>().swap(P.inv_ops);
   std::vector<uint32_t>().swap(P.inv_rec);
   std::vector<double>().swap(P.rec_p);
@@ -609,8 +613,8 @@ int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* o) {
       r[2] = C.tabl ? 1 : 0;
       r[3] = C.grid;
       r[4] = (int64_t)C.lds();
-      r[5] = C.L.cap;
-      r[6] = C.L.rcap;
+      r[5] = C.L.shared_bytes;
+      r[6] = 0;
       r[7] = C.L.wave_bytes;
     }
   return WHALE_OK;
@@ -640,24 +644,24 @@ namespace {
 // hold at most `cap` items and `rcap` records.  The slice tables go to LDS
 // whenever they fit; then the shape with the most resident waves per compute
 // unit wins (LDS-limited, at most kWavesPerCU).
-constexpr int kWavesPerCU = 16;
+constexpr int kWavesPerCU = 8;
 
-bool plan_launch(const whale_model* m, bool fwd, int mode, uint32_t cap, uint32_t rcap, uint64_t nfam, bool multi,
-                 LaunchCfg* out) {
-  cap = (std::max(cap, 1u) + 15) & ~15u;
-  rcap = (std::max(rcap, 4u) + 3) & ~3u;
+bool plan_launch(const whale_model* m, bool fwd, int mode, size_t wave, uint64_t nfam, bool multi, LaunchCfg* out) {
+  wave = align16(std::max<size_t>(wave, 16));
   const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
-  const size_t wave = fwd ? fwd_wave_bytes(mode, cap, rcap) : bwd_wave_bytes(mode, cap, rcap);
-  const size_t shared = fwd ? fwd_shared_bytes(mode, m->h.tab_rows) : bwd_shared_bytes(mode, m->h.tab_rows, AL.n);
+  auto shared_of = [&](bool tabl) {
+    return fwd ? fwd_shared_bytes(mode, tabl, m->h.tab_rows, m->h.n, m->h.n_wgd)
+               : bwd_shared_bytes(mode, tabl, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd);
+  };
   struct Cand {
     int fpw;
     bool tabl;
-  } const cands[3] = {{4, true}, {1, true}, {1, false}};
+  } const cands[4] = {{8, true}, {4, true}, {1, true}, {1, false}};
   int best = -1, best_waves = 0, best_wg = 0;
-  for (int j = 0; j < 3; ++j) {
+  for (int j = 0; j < 4; ++j) {
     if (cands[j].fpw > 1 && !multi) continue;
     if (best >= 0 && cands[best].tabl && !cands[j].tabl) break;
-    const size_t lds = (cands[j].tabl ? shared : 0) + (size_t)cands[j].fpw * wave;
+    const size_t lds = shared_of(cands[j].tabl) + (size_t)cands[j].fpw * wave;
     if (lds > m->ctx->lds_wg) continue;
     const int wg = (int)std::min<size_t>(m->ctx->lds_cu / lds, (size_t)(kWavesPerCU / cands[j].fpw));
     if (wg * cands[j].fpw > best_waves) {
@@ -670,41 +674,49 @@ bool plan_launch(const whale_model* m, bool fwd, int mode, uint32_t cap, uint32_
   LaunchCfg C;
   C.fpw = cands[best].fpw;
   C.tabl = cands[best].tabl;
-  C.L.cap = cap;
-  C.L.rcap = rcap;
+  C.waves_per_cu = best_waves;
   C.L.wave_bytes = (uint32_t)wave;
-  C.L.shared_bytes = C.tabl ? (uint32_t)shared : 0u;
+  C.L.shared_bytes = (uint32_t)shared_of(C.tabl);
   const uint64_t need = (nfam + C.fpw - 1) / C.fpw;
   C.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)m->ctx->n_cu * best_wg, need));
   *out = C;
   return true;
 }
 
+// a launch over the families `fams`: the wave region fits the largest of
+// them, with the root records staged in LDS unless reading them from global
+// memory lets more waves share a compute unit
+bool plan_families(const whale_batch* b, bool fwd, int mode, const uint32_t* fams, size_t nf, uint64_t grid_fams,
+                   bool multi, LaunchCfg* out) {
+  size_t with = 16, without = 16;
+  for (size_t j = 0; j < nf; ++j) {
+    const FamDev& d = b->plan.fam[fams[j]];
+    with = std::max(with, fam_lds(d, mode, fwd, true));
+    without = std::max(without, fam_lds(d, mode, fwd, false));
+  }
+  LaunchCfg a, c;
+  const bool oa = plan_launch(b->model, fwd, mode, with, grid_fams, multi, &a);
+  const bool oc = plan_launch(b->model, fwd, mode, without, grid_fams, multi, &c);
+  if (oa && (!oc || a.waves_per_cu >= c.waves_per_cu)) *out = a;
+  else if (oc) *out = c;
+  return oa || oc;
+}
+
 // launch shapes of every class, mode and direction, and of the FP64 pass over
 // families the FP32 form could not represent
 int plan_batch_launches(whale_batch* b) {
   const whale_model* m = b->model;
-  const PlanHost& P = b->plan;
   for (int mode = 0; mode < 3; ++mode)
     for (int dir = 0; dir < 2; ++dir)
       for (int k = 0; k < whale_batch::kClasses; ++k) {
-        const uint64_t nf = b->cls_hi[k] - b->cls_lo[k];
-        uint32_t rcap = b->cls_rcap[k];
-        bool ok = plan_launch(m, dir == 0, mode, b->cls_cap[k], rcap, nf, true, &b->cfg[mode][dir][k]);
-        if (!ok && k == 2) {  // root stages then read their records from global memory
-          rcap = std::max(P.max_recs, 4u);
-          ok = plan_launch(m, dir == 0, mode, b->cls_cap[k], rcap, nf, true, &b->cfg[mode][dir][k]);
-        }
-        if (!ok) return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
+        const uint32_t lo = b->cls_lo[k], hi = b->cls_hi[k];
+        if (!plan_families(b, dir == 0, mode, b->order_host.data() + lo, hi - lo, hi - lo, true, &b->cfg[mode][dir][k]))
+          return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
       }
-  for (int dir = 0; dir < 2; ++dir) {
-    bool ok = plan_launch(m, dir == 0, WHALE_FP64_SCALED, b->cls_cap[2], b->cls_rcap[2], (uint64_t)m->ctx->n_cu, false,
-                          &b->cfg_fb[dir]);
-    if (!ok)
-      ok = plan_launch(m, dir == 0, WHALE_FP64_SCALED, b->cls_cap[2], std::max(P.max_recs, 4u),
-                       (uint64_t)m->ctx->n_cu, false, &b->cfg_fb[dir]);
-    if (!ok) return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
-  }
+  for (int dir = 0; dir < 2; ++dir)
+    if (!plan_families(b, dir == 0, WHALE_FP64_SCALED, b->order_host.data(), b->order_host.size(),
+                       (uint64_t)m->ctx->n_cu, false, &b->cfg_fb[dir]))
+      return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
   return WHALE_OK;
 }
 
@@ -755,10 +767,8 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     return WHALE_OK;
   };
   auto single = [&](int dir, uint32_t f, const WorkRaw& Wf) -> int {
-    const FamDev& fd = b->plan.fam[f];
     LaunchCfg C;
-    if (!plan_launch(m, dir == 0, mode, fd.max_items, std::max(fd.max_recs, fd.root_recs), 1, false, &C) &&
-        !plan_launch(m, dir == 0, mode, fd.max_items, fd.max_recs, 1, false, &C))
+    if (!plan_families(b, dir == 0, mode, &f, 1, 1, false, &C))
       return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
     C.grid = 1;
     if (dir == 0) HIPCHK(launch_forward(mode, store_all, C, st, B, M, Wf, (int32_t)f, 0, 0));

@@ -21,6 +21,7 @@ struct BatchDev {
   const int32_t* it_bar;
   const uint32_t* rec_ops;
   const uint32_t* rec_sid;
+  const uint16_t* rec_own;
   const uint32_t* inv_ops;
   const uint32_t* inv_rec;
   const double* rec_p;
@@ -72,12 +73,36 @@ struct AccLayout {
 
 __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
+// LDS of one wave, laid out afresh for every stage from its item count n and
+// record count nr (s = bytes per Real).  The same functions size the launches
+// on the host and carve the buffers on the device.
+//   forward stage:  bA bB bF sAcc [n] | sCoef sP [nr] | sOps [nr] | sOwn [nr] (u16) |
+//                   sRO [n+1] | sSegL [n] (u16) | sEi [n] | sFlag [n] (scaled forms)
+//   forward root:   bA bF bB [n] | sCoef [nr] | sOps [nr] | sRO [n+1] | sEi [n] (scaled)
+//   reverse stage:  bA bB bV bG sFo [n] | sCoef [nr] | sOps [nr] | sOwn [nr] (u16) |
+//                   sRO [n+1] | sSegL [n] (u16)
+//   reverse root:   bA bB bV bG [n] | sCoef [nr] | sOps [nr] | sRO [n+1]
+// (root records are staged only when they fit: nr = 0 reads them from global memory)
+__host__ __device__ inline size_t fwd_stage_bytes(size_t s, bool scaled, size_t n, size_t nr) {
+  return align16(4 * n * s) + align16(2 * nr * s) + align16(4 * nr) + align16(2 * nr) + align16(4 * (n + 1)) +
+         align16(2 * n) + (scaled ? align16(4 * n) + align16(n) : 0);
+}
+__host__ __device__ inline size_t fwd_root_bytes(size_t s, bool scaled, size_t n, size_t nr) {
+  return align16(3 * n * s) + align16(nr * s) + align16(4 * nr) + align16(4 * (n + 1)) + (scaled ? align16(4 * n) : 0);
+}
+__host__ __device__ inline size_t bwd_stage_bytes(size_t s, size_t n, size_t nr) {
+  return align16(5 * n * s) + align16(nr * s) + align16(4 * nr) + align16(2 * nr) + align16(4 * (n + 1)) +
+         align16(2 * n);
+}
+__host__ __device__ inline size_t bwd_root_bytes(size_t s, size_t n, size_t nr) {
+  return align16(4 * n * s) + align16(nr * s) + align16(4 * nr) + align16(4 * (n + 1));
+}
+
 // LDS of one launch of k_forward / k_backward: every wave of a workgroup runs
-// its own family in `wave_bytes` of LDS holding `cap` items and `rcap` records
-// of a stage; `shared_bytes` (slice tables, reverse-sweep accumulators) come
-// first and are shared by the workgroup's waves.
+// its own family in `wave_bytes` of LDS (laid out per stage); `shared_bytes`
+// (slice tables, reverse-sweep accumulators, node constants) come first and
+// are shared by the workgroup's waves.
 struct LdsCfg {
-  uint32_t cap, rcap;
   uint32_t wave_bytes;
   uint32_t shared_bytes;
 };
@@ -86,15 +111,14 @@ struct LdsCfg {
 struct LaunchCfg {
   int grid = 1;
   int fpw = 1;
+  int waves_per_cu = 0;  // resident waves per compute unit the LDS allows
   bool tabl = false;
   LdsCfg L{};
   size_t lds() const { return (size_t)L.shared_bytes + (size_t)fpw * L.wave_bytes; }
 };
 
-size_t fwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap);
-size_t bwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap);
-size_t fwd_shared_bytes(int mode, int tab_rows);
-size_t bwd_shared_bytes(int mode, int tab_rows, int acc_n);
+size_t fwd_shared_bytes(int mode, bool tabl, int tab_rows, int n, int nwgd);
+size_t bwd_shared_bytes(int mode, bool tabl, int tab_rows, int acc_n, int n, int nwgd);
 
 // forward/backward launch: `mode` is WHALE_FP64_PLAIN / WHALE_FP32_SCALED /
 // WHALE_FP64_SCALED; fam_single >= 0 runs that one family (first wave of a

@@ -219,6 +219,7 @@ struct FamPlan {
   std::vector<uint32_t> it_seg, it_gid, it_rec, it_inv;  // it_seg family-relative, rec/inv family-relative
   std::vector<int32_t> it_cf, it_cg, it_bar;
   std::vector<uint32_t> rec_ops, rec_sid, inv_ops, inv_rec;  // inv_rec family-relative
+  std::vector<uint16_t> rec_own;                              // stage-local item owning each record
   std::vector<uint32_t> lvl;
   uint64_t ell = 0, ibar = 0;
   int64_t cells = 0, split_terms = 0, sum_compat = 0, row_steps = 0;
@@ -349,7 +350,7 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
     }
     sd.n_items = local;
     if (local > 0xffff) { F.err = "stage with more than 65535 items"; return sd; }
-    F.max_items = std::max(F.max_items, local);
+    if (!is_root) F.max_items = std::max(F.max_items, local);
     std::vector<std::vector<uint32_t>> inv_p(local), inv_s(local), inv_r(local);
     for (int e : segs) {
       const SegDev& s = F.seg[segof[e]];
@@ -388,6 +389,7 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
           uint32_t r = (uint32_t)F.rec_ops.size();
           F.rec_ops.push_back(l1 | (l2 << 16));
           F.rec_sid.push_back((uint32_t)(k - soff[0]));
+          F.rec_own.push_back((uint16_t)me);
           inv_p[l1].push_back(me); inv_s[l1].push_back(l2); inv_r[l1].push_back(r);
           inv_p[l2].push_back(me); inv_s[l2].push_back(l1); inv_r[l2].push_back(r);
         }
@@ -509,6 +511,7 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
     for (size_t r = 0; r < F.rec_ops.size(); ++r) {
       P.rec_ops.push_back(F.rec_ops[r]);
       P.rec_sid.push_back(F.rec_sid[r]);
+      P.rec_own.push_back(F.rec_own[r]);
       P.rec_p.push_back(p[d.split0 + F.rec_sid[r]]);
     }
     for (size_t k = 0; k < F.inv_ops.size(); ++k) {
@@ -518,7 +521,7 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
     for (auto v : F.lvl) P.lvl.push_back(v);
     ell += F.ell;
     ibar += F.ibar;
-    P.max_items = std::max(P.max_items, F.max_items);
+    P.max_items = std::max(P.max_items, std::max(F.max_items, F.n_clades));
     P.max_recs = std::max(P.max_recs, F.max_recs);
     P.max_root_recs = std::max(P.max_root_recs, F.root_recs);
     P.cells += F.cells;

@@ -278,6 +278,88 @@ __device__ __forceinline__ void slice(const ModelDev& M, const Real* sTab, uint3
 }
 
 // ---------------------------------------------------------------------------
+// the segments of a stage, one per lane
+// ---------------------------------------------------------------------------
+// Row 0 and the register caches need each item's segment and node.  A stage's
+// segments are loaded once, lane s holding segment s, and items fetch theirs
+// with a lane shuffle instead of a chain of dependent global loads.  Stages of
+// more than 64 segments read them from global memory.
+struct SegInfo {
+  uint32_t ell, nc, R, tab, item0, ibar;
+  int32_t node, kind, c0, c1, wgd;
+};
+
+__device__ __forceinline__ SegInfo seg_info(const BatchDev& B, const ModelDev& M, uint32_t si) {
+  const SegDev sg = B.seg[si];
+  const NodeDev& nd = M.node[sg.node];
+  SegInfo t;
+  t.ell = sg.ell;
+  t.nc = sg.nc;
+  t.R = sg.R;
+  t.tab = (uint32_t)nd.tab;
+  t.item0 = sg.item0;
+  t.ibar = sg.ibar;
+  t.node = (int32_t)sg.node;
+  t.kind = nd.kind;
+  t.c0 = nd.child0;
+  t.c1 = nd.child1;
+  t.wgd = nd.wgd;
+  return t;
+}
+
+struct SegTab {
+  SegInfo mine;
+  uint32_t seg0;
+  bool inreg;
+  __device__ void load(const BatchDev& B, const ModelDev& M, uint32_t seg0_, uint32_t nsg, uint32_t lane) {
+    seg0 = seg0_;
+    inreg = nsg <= (uint32_t)kWG;
+    if (inreg && lane < nsg) mine = seg_info(B, M, seg0 + lane);
+  }
+  // segment `sl` (stage-local); every lane of the wave must call this together
+  __device__ __forceinline__ SegInfo get(const BatchDev& B, const ModelDev& M, uint32_t sl) const {
+    if (!inreg) return seg_info(B, M, seg0 + sl);
+    const int src = (int)sl;
+    SegInfo t;
+    t.ell = __shfl(mine.ell, src, kWG);
+    t.nc = __shfl(mine.nc, src, kWG);
+    t.R = __shfl(mine.R, src, kWG);
+    t.tab = __shfl(mine.tab, src, kWG);
+    t.item0 = __shfl(mine.item0, src, kWG);
+    t.ibar = __shfl(mine.ibar, src, kWG);
+    t.node = __shfl(mine.node, src, kWG);
+    t.kind = __shfl(mine.kind, src, kWG);
+    t.c0 = __shfl(mine.c0, src, kWG);
+    t.c1 = __shfl(mine.c1, src, kWG);
+    t.wgd = __shfl(mine.wgd, src, kWG);
+    return t;
+  }
+};
+
+// per-node constants of the current parameters in LDS: ε at the top of each
+// branch, leafℙ, and q per WGD
+template <typename Real>
+struct NodeC {
+  Real *etop, *leafP, *q;
+  __device__ NodeC(unsigned char* base, int n) {
+    etop = reinterpret_cast<Real*>(base);
+    leafP = etop + n;
+    q = leafP + n;
+  }
+  __device__ void fill(const ModelDev& M, int tid, int nthreads) {
+    for (int e = tid; e < M.n; e += nthreads) {
+      etop[e] = (Real)eps_top(M, e);
+      leafP[e] = (Real)M.node[e].leafP;
+    }
+    for (int w = tid; w < M.n_wgd; w += nthreads) q[w] = (Real)M.par[3 * M.n + w];
+  }
+};
+
+__host__ __device__ inline size_t nodec_bytes(size_t real, int n, int nwgd) {
+  return align16((2 * (size_t)n + (size_t)(nwgd > 0 ? nwgd : 1)) * real);
+}
+
+// ---------------------------------------------------------------------------
 // row-invariant data of one item, cached in registers for a stage's rows
 // ---------------------------------------------------------------------------
 constexpr int kCache = 8;  // items per lane held in registers (stages of up to 512 items)
@@ -300,23 +382,81 @@ __device__ __forceinline__ ItemRow load_item(const BatchDev& B, const ModelDev& 
   return it;
 }
 
-// LDS of one wave (forward):  bA bB bF [cap] sCoef [rcap] (Real) | sOps [rcap] |
-// sRO [cap+1] (record offsets) | sEi [cap] (exponents) sFlag [cap] (scaled forms)
-template <typename Real, bool Scaled>
-struct FwdLds {
-  Real *bA, *bB, *bF, *sCoef;
+__device__ __forceinline__ ItemRow item_row(const SegInfo& sg, uint32_t i) {
+  ItemRow it;
+  it.cell = sg.ell + (i - sg.item0);
+  it.stride = sg.nc;
+  it.tab = sg.tab;
+  it.R = sg.R;
+  return it;
+}
+
+// clade-size level boundaries of a family's root stage: the first 128 held
+// in two registers across the wave and read with readlane (uniform level)
+struct LevelBounds {
+  uint32_t lo, hi;
+  __device__ LevelBounds(const BatchDev& B, const FamDev& F, uint32_t lane) {
+    lo = lane <= F.n_lvl ? B.lvl[F.lvl0 + lane] : 0u;
+    hi = lane + kWG <= F.n_lvl ? B.lvl[F.lvl0 + kWG + lane] : 0u;
+  }
+  __device__ __forceinline__ uint32_t at(const BatchDev& B, const FamDev& F, uint32_t L) const {
+    if (L < (uint32_t)kWG) return (uint32_t)__builtin_amdgcn_readlane((int)lo, (int)L);
+    if (L < 2u * kWG) return (uint32_t)__builtin_amdgcn_readlane((int)hi, (int)(L - kWG));
+    return B.lvl[F.lvl0 + L];
+  }
+};
+
+// LDS of one wave for a forward stage of n items and nr records (the layout of
+// fwd_stage_bytes) and for the root (fwd_root_bytes)
+template <typename Real>
+struct FwdStage {
+  Real *bA, *bB, *bF, *sAcc, *sCoef, *sP;
   uint32_t *sOps, *sRO;
+  uint16_t *sOwn, *sSegL;
   int32_t* sEi;
   uint8_t* sFlag;
-  __device__ FwdLds(unsigned char* wb, uint32_t cap, uint32_t rcap) {
-    bA = reinterpret_cast<Real*>(wb);
-    bB = bA + cap;
-    bF = bB + cap;
-    sCoef = bF + cap;
-    sOps = reinterpret_cast<uint32_t*>(wb + align16((3 * (size_t)cap + rcap) * sizeof(Real)));
-    sRO = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(sOps) + align16(4 * (size_t)rcap));
-    sEi = reinterpret_cast<int32_t*>(reinterpret_cast<unsigned char*>(sRO) + align16(4 * ((size_t)cap + 1)));
-    sFlag = reinterpret_cast<uint8_t*>(sEi + cap);
+  __device__ FwdStage(unsigned char* p, size_t n, size_t nr) {
+    const size_t s = sizeof(Real);
+    bA = reinterpret_cast<Real*>(p);
+    bB = bA + n;
+    bF = bB + n;
+    sAcc = bF + n;
+    p += align16(4 * n * s);
+    sCoef = reinterpret_cast<Real*>(p);
+    sP = sCoef + nr;
+    p += align16(2 * nr * s);
+    sOps = reinterpret_cast<uint32_t*>(p);
+    p += align16(4 * nr);
+    sOwn = reinterpret_cast<uint16_t*>(p);
+    p += align16(2 * nr);
+    sRO = reinterpret_cast<uint32_t*>(p);
+    p += align16(4 * (n + 1));
+    sSegL = reinterpret_cast<uint16_t*>(p);
+    p += align16(2 * n);
+    sEi = reinterpret_cast<int32_t*>(p);
+    p += align16(4 * n);
+    sFlag = reinterpret_cast<uint8_t*>(p);
+  }
+};
+
+template <typename Real>
+struct FwdRoot {
+  Real *bA, *bF, *bB, *sCoef;
+  uint32_t *sOps, *sRO;
+  int32_t* sEi;
+  __device__ FwdRoot(unsigned char* p, size_t n, size_t nr) {
+    const size_t s = sizeof(Real);
+    bA = reinterpret_cast<Real*>(p);
+    bF = bA + n;
+    bB = bF + n;
+    p += align16(3 * n * s);
+    sCoef = reinterpret_cast<Real*>(p);
+    p += align16(nr * s);
+    sOps = reinterpret_cast<uint32_t*>(p);
+    p += align16(4 * nr);
+    sRO = reinterpret_cast<uint32_t*>(p);
+    p += align16(4 * (n + 1));
+    sEi = reinterpret_cast<int32_t*>(p);
   }
 };
 
@@ -324,7 +464,7 @@ struct FwdLds {
 // forward
 // ---------------------------------------------------------------------------
 template <typename Real, bool Scaled, bool StoreAll, int FPW, bool TabL>
-__global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+__global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                        const uint32_t* fam_count) {
   if (fam_count) fam_end = fam_begin + *fam_count;
@@ -332,23 +472,16 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
   const uint32_t lane = threadIdx.x % kWG;
   const uint32_t wv = threadIdx.x / kWG;
   Real* sTab = reinterpret_cast<Real*>(smem);
+  NodeC<Real> NC(smem + (TabL ? align16(2 * (size_t)M.tab_rows * sizeof(Real)) : 0), M.n);
   if (TabL) {
     for (int r = threadIdx.x; r < M.tab_rows; r += kWG * FPW) {
       sTab[2 * r] = (Real)M.phi[r];
       sTab[2 * r + 1] = (Real)M.psi[r];
     }
-    __syncthreads();
   }
-  const uint32_t cap = L.cap, rcap = L.rcap;
-  FwdLds<Real, Scaled> S_(smem + L.shared_bytes + (size_t)wv * L.wave_bytes, cap, rcap);
-  Real* const bA = S_.bA;
-  Real* const bB = S_.bB;
-  Real* const bF = S_.bF;
-  Real* const sCoef = S_.sCoef;
-  uint32_t* const sOps = S_.sOps;
-  uint32_t* const sRO = S_.sRO;
-  int32_t* const sEi = S_.sEi;
-  uint8_t* const sFlag = S_.sFlag;
+  NC.fill(M, threadIdx.x, kWG * FPW);
+  __syncthreads();
+  unsigned char* const wb = smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
   bool first = true;
   for (;;) {
     uint32_t f;
@@ -360,7 +493,7 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
       if (lane == 0) k = atomicAdd(W.ticket, 1u);
       k = fam_begin + __builtin_amdgcn_readfirstlane(k);
       if (k >= fam_end) break;
-      f = B.order[k];
+      f = __builtin_amdgcn_readfirstlane(B.order[k]);  // wave-uniform: family data in scalar registers
     }
     first = false;
     const FamDev F = B.fam[f];
@@ -373,13 +506,30 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
       const StageDev S = B.stage[F.stage0 + s];
       const uint32_t n = S.n_items, i0 = S.item0;
       const uint32_t rb = B.it_rec[i0];
-      const uint32_t nrec = B.it_rec[i0 + n] - rb;  // <= rcap (size class)
+      const uint32_t nrec = B.it_rec[i0 + n] - rb;
+      const FwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
+      Real* const bA = T.bA;
+      Real* const bB = T.bB;
+      Real* const bF = T.bF;
+      Real* const sAcc = T.sAcc;
+      Real* const sCoef = T.sCoef;
+      Real* const sP = T.sP;
+      uint32_t* const sOps = T.sOps;
+      uint16_t* const sOwn = T.sOwn;
+      uint32_t* const sRO = T.sRO;
+      uint16_t* const sSegL = T.sSegL;
+      int32_t* const sEi = T.sEi;
+      uint8_t* const sFlag = T.sFlag;
       wave_sync_mem();  // ℓ and exponents of the stages below
+      SegTab ST;
+      ST.load(B, M, S.seg0, S.n_segs, lane);
       for (uint32_t i = lane; i < n; i += kWG) {
         const uint32_t ia = i0 + i;
         const int32_t cf = B.it_cf[ia], cg = B.it_cg[ia];
+        sSegL[i] = (uint16_t)(B.it_seg[ia] - S.seg0);
         bF[i] = cf >= 0 ? ell[cf] : Real(0);
         bB[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops (bB is free until row 1)
+        sAcc[i] = 0;
         sRO[i] = B.it_rec[ia] - rb;
         if (Scaled) {
           const int e = E[B.it_gid[ia]];
@@ -388,38 +538,40 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
         }
       }
       if (lane == 0) sRO[n] = nrec;
+#pragma unroll 4
       for (uint32_t k = lane; k < nrec; k += kWG) {
         sOps[k] = B.rec_ops[rb + k];
-        sCoef[k] = (Real)B.rec_p[rb + k];  // p itself; the scaled forms rescale it after row 0
+        sOwn[k] = B.rec_own[rb + k];
+        const Real p = (Real)B.rec_p[rb + k];
+        sCoef[k] = p;  // the scaled forms rescale it after row 0
+        sP[k] = p;
       }
       wave_sync();
       // row 0: terms from the child branches (whale!/whalewgd!, src/core.jl:95-128)
-      for (uint32_t i = lane; i < n; i += kWG) {
-        const uint32_t ia = i0 + i;
-        const SegDev sg = B.seg[B.it_seg[ia]];
-        const NodeDev& nd = M.node[sg.node];
+      auto row0 = [&](const SegInfo& sg, uint32_t i) {
         const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
         Real v = 0;
-        if (nd.kind == WHALE_NODE_LEAF) {
-          if (B.clade_size[F.clade0 + B.it_gid[ia]] == 1) {
+        if (sg.kind == WHALE_NODE_LEAF) {
+          if (r1 == r0) {  // a clade of one gene (every larger clade has a split)
+            const Real lp = NC.leafP[sg.node];
             if (Scaled) {
               const int Eg = sEi[i];
               if (Eg == kUndef) {
                 int ex;
-                v = frexp_r((Real)nd.leafP, &ex);
+                v = frexp_r(lp, &ex);
                 sEi[i] = ex;
                 sFlag[i] = kYoung;
               } else {
-                v = ldexp_r((Real)nd.leafP, -Eg);
+                v = ldexp_r(lp, -Eg);
               }
             } else {
-              v = (Real)nd.leafP;
+              v = lp;
             }
           }
         } else if (!Scaled) {
-          if (nd.kind == WHALE_NODE_WGD) {
-            const Real qq = (Real)M.par[3 * M.n + nd.wgd];
-            const Real ef = (Real)eps_top(M, nd.child0);
+          if (sg.kind == WHALE_NODE_WGD) {
+            const Real qq = NC.q[sg.wgd];
+            const Real ef = NC.etop[sg.c0];
             Real r = 0;
             for (uint32_t k = r0; k < r1; ++k) {
               const uint32_t ops = sOps[k];
@@ -427,7 +579,7 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
             }
             v = r * qq + ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]);
           } else {
-            const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
+            const Real ef = NC.etop[sg.c0], eg = NC.etop[sg.c1];
             Real sp = 0;
             for (uint32_t k = r0; k < r1; ++k) {
               const uint32_t ops = sOps[k];
@@ -438,10 +590,10 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
           }
         } else {
           // scaled: explicit exponents (the operands come from other segments)
-          const bool wgd = nd.kind == WHALE_NODE_WGD;
-          const Real qq = wgd ? (Real)M.par[3 * M.n + nd.wgd] : Real(1);
-          const Real ef = (Real)eps_top(M, nd.child0);
-          const Real eg = wgd ? Real(0) : (Real)eps_top(M, nd.child1);
+          const bool wgd = sg.kind == WHALE_NODE_WGD;
+          const Real qq = wgd ? NC.q[sg.wgd] : Real(1);
+          const Real ef = NC.etop[sg.c0];
+          const Real eg = wgd ? Real(0) : NC.etop[sg.c1];
           const int Eg = sEi[i];
           XSum<Real> xs;
           for (uint32_t k = r0; k < r1; ++k) {
@@ -470,6 +622,24 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
         }
         bA[i] = v;
         if (StoreAll || sg.R == 1) ell[sg.ell + (i - sg.item0)] = v;
+      };
+      // row 0 of every item; the first kCache per lane also fill the register cache
+      ItemRow cache[kCache];
+#pragma unroll
+      for (int c = 0; c < kCache; ++c) {
+        if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
+          const uint32_t i = lane + c * kWG;
+          const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+          if (i < n) {
+            cache[c] = item_row(sg, i);
+            row0(sg, i);
+          }
+        }
+      }
+      for (uint32_t base = kCache * kWG; base < n; base += kWG) {
+        const uint32_t i = base + lane;
+        const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+        if (i < n) row0(sg, i);
       }
       wave_sync();
       // scaled forms: coefficients for the exponents now fixed.  A record whose
@@ -479,7 +649,7 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
           const int Eg = sEi[i];
           for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
             const uint32_t ops = sOps[k];
-            const Real p = (Real)B.rec_p[rb + k];
+            const Real p = sP[k];
             const Real c = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
             sCoef[k] = Eg == kUndef ? p : c;
             if (StoreAll) W.coef[rb + k] = c;
@@ -493,14 +663,18 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
       // rows 1..R-1: within_branch! (src/core.jl:83-92)
       Real* cur = bA;
       Real* nxt = bB;
-      ItemRow cache[kCache];
-#pragma unroll
-      for (int c = 0; c < kCache; ++c) {
-        const uint32_t i = lane + c * kWG;
-        if (i < n) cache[c] = load_item(B, M, i0 + i, i, sRO);
-      }
       for (uint32_t r = 1; r < S.r_max; ++r) {
         bool dirty = false;
+        // split sums of every item, one record per lane (lane-balanced however
+        // the records are spread over the items)
+#pragma unroll 2
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          const uint32_t ops = sOps[k];
+          const uint32_t own = sOwn[k];
+          const Real t = sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
+          if (t != Real(0)) lds_add(&sAcc[own], t);
+        }
+        wave_sync();
         auto update = [&](const ItemRow& it, uint32_t i) {
           Real v = cur[i];
           if (r < it.R) {
@@ -509,12 +683,7 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
             const uint8_t fl = Scaled ? sFlag[i] : 0;
             const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
             if (!Scaled || !(fl & kPending) || r1 == r0) {
-              Real acc = 0;
-              for (uint32_t k = r0; k < r1; ++k) {
-                const uint32_t ops = sOps[k];
-                acc += sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
-              }
-              v = ph * v + ps * acc;
+              v = ph * v + ps * sAcc[i];
               if (Scaled && sizeof(Real) == sizeof(float) && (fl & kYoung) &&
                   fabsf((float)v) > (float)(1ull << kBump) && isfinite((float)v)) {
                 // renormalise a young clade in place (its only values are this branch's rows);
@@ -557,6 +726,7 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
             if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
           }
           nxt[i] = v;
+          sAcc[i] = 0;
         };
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
@@ -596,9 +766,18 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
       const double xi = 1.0 - (1.0 - eta) * er;
       const Real cA = (Real)((1.0 - eta) * xi / eta);
       const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
-      const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
+      const Real ef = NC.etop[nd.child0], eg = NC.etop[nd.child1];
       const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
-      const bool inl = nrec <= rcap;  // records staged in LDS
+      // records staged in LDS when they fit the wave's region
+      const bool inl = fwd_root_bytes(sizeof(Real), Scaled, n, nrec) <= L.wave_bytes;
+      const FwdRoot<Real> T(wb, n, inl ? nrec : 0);
+      Real* const bA = T.bA;
+      Real* const bF = T.bF;
+      Real* const bB = T.bB;
+      Real* const sCoef = T.sCoef;
+      uint32_t* const sOps = T.sOps;
+      uint32_t* const sRO = T.sRO;
+      int32_t* const sEi = T.sEi;
       for (uint32_t i = lane; i < n; i += kWG) {
         const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
         bF[i] = cf >= 0 ? ell[cf] : Real(0);
@@ -607,14 +786,17 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
         if (Scaled) sEi[i] = E[i];  // root items are the clades in order
       }
       if (lane == 0) sRO[n] = nrec;
-      if (inl)
+      if (inl) {
+#pragma unroll 4
         for (uint32_t k = lane; k < nrec; k += kWG) {
           sOps[k] = B.rec_ops[rb + k];
           sCoef[k] = (Real)B.rec_p[rb + k];
         }
+      }
+      const LevelBounds LB(B, F, lane);
       wave_sync();
       for (uint32_t Lv = 0; Lv < F.n_lvl; ++Lv) {
-        const uint32_t a0 = B.lvl[F.lvl0 + Lv], a1 = B.lvl[F.lvl0 + Lv + 1];
+        const uint32_t a0 = LB.at(B, F, Lv), a1 = LB.at(B, F, Lv + 1);
         for (uint32_t i = a0 + lane; i < a1; i += kWG) {
           const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
           Real v;
@@ -685,7 +867,8 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
           const int Eg = sEi[i];
           for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
             const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
-            W.coef[rb + k] = coef_of((Real)B.rec_p[rb + k], sEi[ops & 0xffff], sEi[ops >> 16], Eg);
+            const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
+            W.coef[rb + k] = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
           }
         }
       }
@@ -696,21 +879,51 @@ __global__ __launch_bounds__(kWG* FPW, 3) void k_forward(BatchDev B, ModelDev M,
 // ---------------------------------------------------------------------------
 // backward (reverse sweep); requires a forward with StoreAll
 // ---------------------------------------------------------------------------
-// LDS of one wave: bA bB bV bG [cap] sCoef [rcap] (Real) | sOps [rcap] | sRO [cap+1].
-// Shared (TabL): φ, ψ per table row (Real), then the gradient accumulators in
-// the global layout (doubles), flushed to global memory once per workgroup.
+// LDS of one wave for a reverse-sweep stage (bwd_stage_bytes) and for the
+// root (bwd_root_bytes).  Shared by the workgroup: φ, ψ per table row (Real)
+// and the gradient accumulators in the global layout (doubles, flushed to
+// global memory once per workgroup) when TabL, then the node constants.
 template <typename Real>
-struct BwdLds {
+struct BwdStage {
+  Real *bA, *bB, *bV, *bG, *sFo, *sCoef;
+  uint32_t *sOps, *sRO;
+  uint16_t *sOwn, *sSegL;
+  __device__ BwdStage(unsigned char* p, size_t n, size_t nr) {
+    const size_t s = sizeof(Real);
+    bA = reinterpret_cast<Real*>(p);
+    bB = bA + n;
+    bV = bB + n;
+    bG = bV + n;
+    sFo = bG + n;
+    p += align16(5 * n * s);
+    sCoef = reinterpret_cast<Real*>(p);
+    p += align16(nr * s);
+    sOps = reinterpret_cast<uint32_t*>(p);
+    p += align16(4 * nr);
+    sOwn = reinterpret_cast<uint16_t*>(p);
+    p += align16(2 * nr);
+    sRO = reinterpret_cast<uint32_t*>(p);
+    p += align16(4 * (n + 1));
+    sSegL = reinterpret_cast<uint16_t*>(p);
+  }
+};
+
+template <typename Real>
+struct BwdRoot {
   Real *bA, *bB, *bV, *bG, *sCoef;
   uint32_t *sOps, *sRO;
-  __device__ BwdLds(unsigned char* wb, uint32_t cap, uint32_t rcap) {
-    bA = reinterpret_cast<Real*>(wb);
-    bB = bA + cap;
-    bV = bB + cap;
-    bG = bV + cap;
-    sCoef = bG + cap;
-    sOps = reinterpret_cast<uint32_t*>(wb + align16((4 * (size_t)cap + rcap) * sizeof(Real)));
-    sRO = reinterpret_cast<uint32_t*>(reinterpret_cast<unsigned char*>(sOps) + align16(4 * (size_t)rcap));
+  __device__ BwdRoot(unsigned char* p, size_t n, size_t nr) {
+    const size_t s = sizeof(Real);
+    bA = reinterpret_cast<Real*>(p);
+    bB = bA + n;
+    bV = bB + n;
+    bG = bV + n;
+    p += align16(4 * n * s);
+    sCoef = reinterpret_cast<Real*>(p);
+    p += align16(nr * s);
+    sOps = reinterpret_cast<uint32_t*>(p);
+    p += align16(4 * nr);
+    sRO = reinterpret_cast<uint32_t*>(p);
   }
 };
 
@@ -723,16 +936,14 @@ struct ItemTop {
   int32_t wgd;      // q index of a WGD node
 };
 
-__device__ __forceinline__ ItemTop load_top(const BatchDev& B, const ModelDev& M, uint32_t ia, uint32_t i) {
+__device__ __forceinline__ ItemTop item_top(const SegInfo& sg, uint32_t i) {
   ItemTop t;
-  const SegDev sg = B.seg[B.it_seg[ia]];
-  const NodeDev& nd = M.node[sg.node];
   t.ib = sg.ibar == kNone ? kNone : sg.ibar + (i - sg.item0);
   t.nc = sg.nc;
-  t.kind = nd.kind;
-  t.c0 = nd.child0;
-  t.c1 = nd.child1;
-  t.wgd = nd.wgd;
+  t.kind = sg.kind;
+  t.c0 = sg.c0;
+  t.c1 = sg.c1;
+  t.wgd = sg.wgd;
   return t;
 }
 
@@ -745,25 +956,26 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
   const uint32_t lane = threadIdx.x % kWG;
   const uint32_t wv = threadIdx.x / kWG;
   const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
+  const int T2 = 2 * M.tab_rows;
   Real* sTab = reinterpret_cast<Real*>(smem);
-  double* const acc = TabL ? reinterpret_cast<double*>(smem + align16(2 * (size_t)M.tab_rows * sizeof(Real))) : W.acc;
+  const size_t tab_b = align16((size_t)T2 * sizeof(Real));
+  // accumulators: with TabL, d/dφ_r and d/dψ_r in LDS at the working precision
+  // (flushed into the global FP64 sums at the end of every stage), the rest
+  // (ε, q, A, B) in LDS in FP64; otherwise everything goes to global memory
+  Real* const accPS = reinterpret_cast<Real*>(smem + tab_b);
+  double* const accX = TabL ? reinterpret_cast<double*>(smem + 2 * tab_b) - AL.eps : W.acc;
+  NodeC<Real> NC(smem + (TabL ? 2 * tab_b + align16(8 * (size_t)(AL.n - AL.eps)) : 0), M.n);
   if (TabL) {
     for (int r = threadIdx.x; r < M.tab_rows; r += kWG * FPW) {
       sTab[2 * r] = (Real)M.phi[r];
       sTab[2 * r + 1] = (Real)M.psi[r];
     }
-    for (int j = threadIdx.x; j < AL.n; j += kWG * FPW) acc[j] = 0.0;
-    __syncthreads();
+    for (int j = threadIdx.x; j < T2; j += kWG * FPW) accPS[j] = 0;
+    for (int j = AL.eps + threadIdx.x; j < AL.n; j += kWG * FPW) accX[j] = 0.0;
   }
-  const uint32_t cap = L.cap, rcap = L.rcap;
-  BwdLds<Real> S_(smem + L.shared_bytes + (size_t)wv * L.wave_bytes, cap, rcap);
-  Real* bA = S_.bA;  // d logL / d ℓ, current row
-  Real* bB = S_.bB;  // next row (and ī sums at row 0)
-  Real* const bV = S_.bV;  // ℓ of the row below / child0 tops
-  Real* const bG = S_.bG;  // child1 tops
-  Real* const sCoef = S_.sCoef;
-  uint32_t* const sOps = S_.sOps;
-  uint32_t* const sRO = S_.sRO;
+  NC.fill(M, threadIdx.x, kWG * FPW);
+  __syncthreads();
+  unsigned char* const wb = smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
   const Real* coefs = Scaled ? W.coef : nullptr;
   bool first = true;
   for (;;) {
@@ -776,7 +988,7 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
       if (lane == 0) k = atomicAdd(W.ticket, 1u);
       k = fam_begin + __builtin_amdgcn_readfirstlane(k);
       if (k >= fam_end) break;
-      f = B.order[k];
+      f = __builtin_amdgcn_readfirstlane(B.order[k]);  // wave-uniform: family data in scalar registers
     }
     first = false;
     if (Scaled && sizeof(Real) == sizeof(float) && W.flag[f]) continue;  // done by the FP64 pass
@@ -795,9 +1007,18 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
       const double xi = 1.0 - (1.0 - eta) * er;
       const Real cA = (Real)((1.0 - eta) * xi / eta);
       const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
-      const Real ef = (Real)eps_top(M, nd.child0), eg = (Real)eps_top(M, nd.child1);
+      const Real ef = NC.etop[nd.child0], eg = NC.etop[nd.child1];
       const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
-      const bool inl = nrec <= rcap;
+      const bool inl = bwd_root_bytes(sizeof(Real), n, nrec) <= L.wave_bytes;
+      const BwdRoot<Real> T(wb, n, inl ? nrec : 0);
+      Real* const bA = T.bA;
+      Real* const bB = T.bB;
+      Real* const bV = T.bV;
+      Real* const bG = T.bG;
+      Real* const sCoef = T.sCoef;
+      uint32_t* const sOps = T.sOps;
+      uint32_t* const sRO = T.sRO;
+      const LevelBounds LB(B, F, lane);
       for (uint32_t i = lane; i < n; i += kWG) {
         const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
         bV[i] = ell[sg.ell + i];
@@ -824,7 +1045,7 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
         wave_sync();
         // d logL / d ℓ of each root clade, larger clades first, scattered into the parts
         for (int Lv = (int)F.n_lvl - 1; Lv >= 0; --Lv) {
-          const uint32_t a0 = B.lvl[F.lvl0 + Lv], a1 = B.lvl[F.lvl0 + Lv + 1];
+          const uint32_t a0 = LB.at(B, F, (uint32_t)Lv), a1 = LB.at(B, F, (uint32_t)Lv + 1);
           for (uint32_t i = a0 + lane; i < a1; i += kWG) {
             const Real t = cA * bA[i];
             if (t == Real(0)) continue;
@@ -862,10 +1083,10 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
         sF = wave_sum(sF);
         sG = wave_sum(sG);
         if (lane == 0) {
-          acc_add<TabL>(acc + AL.rootA, (double)sA);
-          acc_add<TabL>(acc + AL.rootB, (double)sBc);
-          acc_add<TabL>(acc + AL.eps + nd.child1, (double)cB * (double)sF);
-          acc_add<TabL>(acc + AL.eps + nd.child0, (double)cB * (double)sG);
+          acc_add<TabL>(accX + AL.rootA, (double)sA);
+          acc_add<TabL>(accX + AL.rootB, (double)sBc);
+          acc_add<TabL>(accX + AL.eps + nd.child1, (double)cB * (double)sF);
+          acc_add<TabL>(accX + AL.eps + nd.child0, (double)cB * (double)sG);
         }
         // ī of the child tops: child0 slot takes ℓ of child1 tops and vice versa
         wave_sync();
@@ -899,40 +1120,77 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
       const StageDev S = B.stage[F.stage0 + s];
       const uint32_t n = S.n_items, i0 = S.item0;
       const uint32_t rb = B.it_rec[i0];
-      const uint32_t nrec = B.it_rec[i0 + n] - rb;  // <= rcap (size class)
+      const uint32_t nrec = B.it_rec[i0 + n] - rb;
+      const BwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
+      Real* bA = T.bA;  // d logL / d ℓ, current row
+      Real* bB = T.bB;  // next row (and ī sums at row 0)
+      Real* const bV = T.bV;  // ℓ of the row below / child0 tops
+      Real* const bG = T.bG;  // child1 tops; ψ·ā per item in the row loop
+      Real* const sFo = T.sFo;  // split sums Σ c·ℓ(γ1)ℓ(γ2) per item (row loop)
+      Real* const sCoef = T.sCoef;
+      uint32_t* const sOps = T.sOps;
+      uint16_t* const sOwn = T.sOwn;
+      uint32_t* const sRO = T.sRO;
+      uint16_t* const sSegL = T.sSegL;
       wave_sync_mem();  // ī written by the stage above
+      SegTab ST;
+      ST.load(B, M, S.seg0, S.n_segs, lane);
       for (uint32_t i = lane; i < n; i += kWG) {
         bA[i] = ibar[B.it_bar[i0 + i]];
+        bV[i] = 0;
         sRO[i] = B.it_rec[i0 + i] - rb;
+        sSegL[i] = (uint16_t)(B.it_seg[i0 + i] - S.seg0);
       }
       if (lane == 0) sRO[n] = nrec;
+#pragma unroll 4
       for (uint32_t k = lane; k < nrec; k += kWG) {
         sOps[k] = B.rec_ops[rb + k];
+        sOwn[k] = B.rec_own[rb + k];
         sCoef[k] = coef_at(rb + k);
       }
       wave_sync();
       ItemRow cache[kCache];
+      ItemTop tops[kCache];
       Real pre[kCache];  // ℓ of the next row down, loaded one row ahead
       const int rtop = (int)S.r_max - 1;
 #pragma unroll
       for (int c = 0; c < kCache; ++c) {
-        const uint32_t i = lane + c * kWG;
         pre[c] = 0;
-        if (i < n) {
-          cache[c] = load_item(B, M, i0 + i, i, sRO);
-          if (rtop >= 1 && rtop < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(rtop - 1) * cache[c].stride];
+        if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
+          const uint32_t i = lane + c * kWG;
+          const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+          if (i < n) {
+            cache[c] = item_row(sg, i);
+            tops[c] = item_top(sg, i);
+            if (rtop >= 1 && rtop < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(rtop - 1) * cache[c].stride];
+          }
         }
       }
+      Real* const sT = bG;  // ψ_r·ā(r) per item, for the record pass
       for (int r = rtop; r >= 1; --r) {
-        // ℓ of row r-1 for the items whose segment reaches row r
+        // items: ℓ of row r-1, the φ part of ā(r-1) and ψ_r·ā(r)
+        auto item_a = [&](const ItemRow& it, uint32_t i, Real pv) {
+          const Real g = bA[i];
+          if (r < (int)it.R) {
+            Real ph, ps;
+            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
+            bV[i] = pv;
+            bB[i] = ph * g;
+            sT[i] = ps * g;
+          } else {
+            bB[i] = g;
+            sT[i] = 0;
+          }
+          sFo[i] = 0;
+        };
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
           const uint32_t i = lane + c * kWG;
-          if (i < n && r < (int)cache[c].R) bV[i] = pre[c];
+          if (i < n) item_a(cache[c], i, pre[c]);
         }
         for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) {
           const ItemRow it = load_item(B, M, i0 + i, i, sRO);
-          if (r < (int)it.R) bV[i] = ell[it.cell + (size_t)(r - 1) * it.stride];
+          item_a(it, i, r < (int)it.R ? ell[it.cell + (size_t)(r - 1) * it.stride] : Real(0));
         }
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
@@ -940,85 +1198,90 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
           if (i < n && r >= 2 && r - 1 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(r - 2) * cache[c].stride];
         }
         wave_sync();
-        // φ part of d logL / d ℓ(row r-1)
-        auto phi_part = [&](const ItemRow& it, uint32_t i) {
-          const Real g = bA[i];
-          if (r < (int)it.R) {
-            Real ph, ps;
-            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
-            bB[i] = ph * g;
-          } else {
-            bB[i] = g;
+        // records, one per lane: the ψ part scattered into the two parts, and
+        // the split sum of the owner (forcing of d/dψ_r)
+#pragma unroll 2
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          const uint32_t ops = sOps[k];
+          const uint32_t a = ops & 0xffff, b = ops >> 16;
+          const uint32_t own = sOwn[k];
+          const Real c = sCoef[k];
+          const Real va = bV[a], vb = bV[b];
+          const Real fo = c * va * vb;
+          if (fo != Real(0)) lds_add(&sFo[own], fo);
+          const Real t = sT[own] * c;
+          if (t != Real(0)) {
+            lds_add(&bB[a], t * vb);
+            lds_add(&bB[b], t * va);
           }
-        };
-#pragma unroll
-        for (int c = 0; c < kCache; ++c) {
-          const uint32_t i = lane + c * kWG;
-          if (i < n) phi_part(cache[c], i);
         }
-        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) phi_part(load_item(B, M, i0 + i, i, sRO), i);
         wave_sync();
-        // ψ part scattered into the parts of each split; d/dφ_r, d/dψ_r per segment
-        // (all lanes take part: segmented reductions)
-        auto psi_part = [&](const ItemRow& it, uint32_t i, bool valid) {
+        // d/dφ_r, d/dψ_r per segment (all lanes take part: segmented reductions)
+        auto item_c = [&](const ItemRow& it, uint32_t i, bool valid) {
           Real cphi = 0, cpsi = 0;
           uint32_t key = kNone;
           if (valid && r < (int)it.R) {
             const Real g = bA[i];
-            Real ph, ps;
-            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
-            const Real t = ps * g;
-            Real fo = 0;
-            const uint32_t r1 = sRO[i + 1];
-            for (uint32_t k = sRO[i]; k < r1; ++k) {
-              const uint32_t ops = sOps[k];
-              const uint32_t a = ops & 0xffff, b = ops >> 16;
-              const Real c = sCoef[k];
-              const Real va = bV[a], vb = bV[b];
-              fo += c * va * vb;
-              if (t != Real(0)) {
-                lds_add(&bB[a], t * c * vb);
-                lds_add(&bB[b], t * c * va);
-              }
-            }
             cphi = g * bV[i];
-            cpsi = g * fo;
+            cpsi = g * sFo[i];
             key = it.tab;
           }
           bool tail;
           seg_scan2(cphi, cpsi, key, &tail);
           if (tail && key != kNone) {
-            acc_add<TabL>(acc + AL.phi + key + r, (double)cphi);
-            acc_add<TabL>(acc + AL.psi + key + r, (double)cpsi);
+            if (TabL) {
+              if (cphi != Real(0)) lds_add(&accPS[AL.phi + key + r], cphi);
+              if (cpsi != Real(0)) lds_add(&accPS[AL.psi + key + r], cpsi);
+            } else {
+              acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
+              acc_add<false>(W.acc + AL.psi + key + r, (double)cpsi);
+            }
           }
         };
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
           if ((uint32_t)c * kWG < n) {
             const uint32_t i = lane + c * kWG;
-            psi_part(cache[c], i, i < n);
+            item_c(cache[c], i, i < n);
           }
         }
         for (uint32_t base = kCache * kWG; base < n; base += kWG) {
           const uint32_t i = base + lane;
-          psi_part(i < n ? load_item(B, M, i0 + i, i, sRO) : cache[0], i, i < n);
+          item_c(i < n ? load_item(B, M, i0 + i, i, sRO) : cache[0], i, i < n);
         }
         wave_sync();
         Real* t = bA;
         bA = bB;
         bB = t;
       }
+      if (TabL) {
+        // move this stage's d/dφ_r, d/dψ_r into the global FP64 sums (each LDS
+        // partial then holds at most one stage's contributions per family)
+        for (uint32_t sl = 0; sl < S.n_segs; ++sl) {
+          uint32_t tab, R;
+          if (ST.inreg) {
+            tab = (uint32_t)__builtin_amdgcn_readlane((int)ST.mine.tab, (int)sl);
+            R = (uint32_t)__builtin_amdgcn_readlane((int)ST.mine.R, (int)sl);
+          } else {
+            const SegInfo g = seg_info(B, M, S.seg0 + sl);
+            tab = g.tab;
+            R = g.R;
+          }
+          for (uint32_t r = 1 + lane; r < R; r += kWG) {
+            const Real a = __hip_atomic_exchange(&accPS[AL.phi + tab + r], Real(0), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            const Real b = __hip_atomic_exchange(&accPS[AL.psi + tab + r], Real(0), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (a != Real(0)) unsafeAtomicAdd(W.acc + AL.phi + tab + r, (double)a);
+            if (b != Real(0)) unsafeAtomicAdd(W.acc + AL.psi + tab + r, (double)b);
+          }
+        }
+      }
       // row 0: adjoints of the child tops, ε and q
       for (uint32_t i = lane; i < n; i += kWG) {
         const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
         bV[i] = cf >= 0 ? ell[cf] : Real(0);
         bG[i] = cg >= 0 ? ell[cg] : Real(0);
-      }
-      ItemTop tops[kCache];
-#pragma unroll
-      for (int c = 0; c < kCache; ++c) {
-        const uint32_t i = lane + c * kWG;
-        if (i < n) tops[c] = load_top(B, M, i0 + i, i);
       }
       wave_sync();
       // pass over all items: cached tops for the first kCache per lane
@@ -1028,26 +1291,30 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
           const uint32_t i = lane + c * kWG;
           if (i < n) fn(tops[c], i);
         }
-        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) fn(load_top(B, M, i0 + i, i), i);
+        for (uint32_t base = kCache * kWG; base < n; base += kWG) {
+          const uint32_t i = base + lane;
+          const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+          if (i < n) fn(item_top(sg, i), i);
+        }
       };
       // ε̄, q̄ and the direct terms of the child0 slot
       each([&](const ItemTop& tp, uint32_t i) {
         const Real g = bA[i];
         if (tp.kind == WHALE_NODE_SPECIATION) {
-          const Real eg = (Real)eps_top(M, tp.c1);
-          acc_add<TabL>(acc + AL.eps + tp.c1, (double)(g * bV[i]));
-          acc_add<TabL>(acc + AL.eps + tp.c0, (double)(g * bG[i]));
+          const Real eg = NC.etop[tp.c1];
+          acc_add<TabL>(accX + AL.eps + tp.c1, (double)(g * bV[i]));
+          acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * bG[i]));
           bB[i] = g * eg;
         } else if (tp.kind == WHALE_NODE_WGD) {
-          const Real qq = (Real)M.par[3 * M.n + tp.wgd];
-          const Real ef = (Real)eps_top(M, tp.c0);
+          const Real qq = NC.q[tp.wgd];
+          const Real ef = NC.etop[tp.c0];
           Real ret = 0;
           for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
             const uint32_t ops = sOps[k];
             ret += sCoef[k] * bV[ops & 0xffff] * bV[ops >> 16];
           }
-          acc_add<TabL>(acc + AL.q + tp.wgd, (double)(g * (ret - bV[i] + Real(2) * ef * bV[i])));
-          acc_add<TabL>(acc + AL.eps + tp.c0, (double)(g * Real(2) * qq * bV[i]));
+          acc_add<TabL>(accX + AL.q + tp.wgd, (double)(g * (ret - bV[i] + Real(2) * ef * bV[i])));
+          acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * Real(2) * qq * bV[i]));
           bB[i] = g * ((Real(1) - qq) + Real(2) * qq * ef);
         }
       });
@@ -1057,7 +1324,7 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
       each([&](const ItemTop& tp, uint32_t i) {
         if (tp.kind != WHALE_NODE_SPECIATION && tp.kind != WHALE_NODE_WGD) return;
         const bool wgd = tp.kind == WHALE_NODE_WGD;
-        const Real g = wgd ? (Real)M.par[3 * M.n + tp.wgd] * bA[i] : bA[i];
+        const Real g = wgd ? NC.q[tp.wgd] * bA[i] : bA[i];
         if (g == Real(0)) return;
         const Real* other = wgd ? bV : bG;
         for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
@@ -1071,7 +1338,7 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
       wave_sync();
       each([&](const ItemTop& tp, uint32_t i) {
         if (tp.kind == WHALE_NODE_SPECIATION || tp.kind == WHALE_NODE_WGD) ibar[tp.ib] = bB[i];
-        if (tp.kind == WHALE_NODE_SPECIATION) bB[i] = bA[i] * (Real)eps_top(M, tp.c0);
+        if (tp.kind == WHALE_NODE_SPECIATION) bB[i] = bA[i] * NC.etop[tp.c0];
       });
       wave_sync();
       // child1 slot of speciation nodes
@@ -1095,8 +1362,12 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
   }
   if (TabL) {
     __syncthreads();
-    for (int j = threadIdx.x; j < AL.n; j += kWG * FPW) {
-      const double v = acc[j];
+    for (int j = threadIdx.x; j < T2; j += kWG * FPW) {
+      const Real v = accPS[j];
+      if (v != Real(0)) unsafeAtomicAdd(W.acc + j, (double)v);
+    }
+    for (int j = AL.eps + threadIdx.x; j < AL.n; j += kWG * FPW) {
+      const double v = accX[j];
       if (v != 0.0) unsafeAtomicAdd(W.acc + j, v);
     }
   }
@@ -1222,22 +1493,13 @@ __global__ void k_fill_f64(double* p, double v, size_t n) {
 // ---------------------------------------------------------------------------
 static size_t real_size(int mode) { return mode == WHALE_FP32_SCALED ? sizeof(float) : sizeof(double); }
 
-size_t fwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap) {
-  size_t b = align16((3 * (size_t)cap + rcap) * real_size(mode)) + align16(4 * (size_t)rcap) +
-             align16(4 * ((size_t)cap + 1));
-  if (mode != WHALE_FP64_PLAIN) b += align16(4 * (size_t)cap) + align16(cap);
-  return b;
+size_t fwd_shared_bytes(int mode, bool tabl, int tab_rows, int n, int nwgd) {
+  return (tabl ? align16(2 * (size_t)tab_rows * real_size(mode)) : 0) + nodec_bytes(real_size(mode), n, nwgd);
 }
 
-size_t bwd_wave_bytes(int mode, uint32_t cap, uint32_t rcap) {
-  return align16((4 * (size_t)cap + rcap) * real_size(mode)) + align16(4 * (size_t)rcap) +
-         align16(4 * ((size_t)cap + 1));
-}
-
-size_t fwd_shared_bytes(int mode, int tab_rows) { return align16(2 * (size_t)tab_rows * real_size(mode)); }
-
-size_t bwd_shared_bytes(int mode, int tab_rows, int acc_n) {
-  return align16(2 * (size_t)tab_rows * real_size(mode)) + align16(8 * (size_t)acc_n);
+size_t bwd_shared_bytes(int mode, bool tabl, int tab_rows, int acc_n, int n, int nwgd) {
+  const size_t tab_b = align16(2 * (size_t)tab_rows * real_size(mode));
+  return (tabl ? 2 * tab_b + align16(8 * (size_t)(acc_n - 2 * tab_rows)) : 0) + nodec_bytes(real_size(mode), n, nwgd);
 }
 
 template <typename K, typename... Args>
@@ -1257,6 +1519,8 @@ static hipError_t fwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev
                               const uint32_t* fam_count) {
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
+  if (C.tabl && C.fpw == 8)
+    return go(k_forward<R, S, A, 8, true>, C.grid, 8, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
   if (C.tabl && C.fpw == 4)
     return go(k_forward<R, S, A, 4, true>, C.grid, 4, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
   if (C.tabl)
@@ -1270,6 +1534,8 @@ static hipError_t bwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev
                               const uint32_t* fam_count) {
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
+  if (C.tabl && C.fpw == 8)
+    return go(k_backward<R, S, 8, true>, C.grid, 8, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
   if (C.tabl && C.fpw == 4)
     return go(k_backward<R, S, 4, true>, C.grid, 4, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
   if (C.tabl)

@@ -50,7 +50,7 @@ struct FamDev {
   uint32_t split0;       // first split (absolute)
   uint32_t lvl0;         // root size-level boundaries lvl[lvl0 .. lvl0 + n_lvl] (root-stage-local items)
   uint32_t n_lvl;
-  uint32_t max_items;    // largest stage, root included
+  uint32_t max_items;    // largest non-root stage (the root stage holds all n_clades)
   uint32_t max_recs;     // most records of any non-root stage
   uint32_t root_seg;     // absolute segment of the root
   uint32_t root_recs;    // records of the root stage (all splits of the family)
@@ -87,6 +87,7 @@ struct SegDev {
 // Records (one per split of an item's clade):
 //   rec_ops[r]  l1 | l2 << 16, stage-local items of the two parts
 //   rec_sid[r]  family-local split id (p = split_p[split0 + sid])
+//   rec_own[r]  stage-local item owning the record (records of an item are contiguous)
 // Inverse records (one per occurrence of an item as a part of a split):
 //   inv_ops[k]  parent | sibling << 16, stage-local items
 //   inv_rec[k]  absolute record of that split
