@@ -125,6 +125,8 @@ def main():
     st = batch.stats()
     log(f"parsed and planned in {time.time() - t0:.1f}s: {st}")
     for r in batch.launches("fp32"):
+        if not r["families"]:
+            continue
         log("launch fp32 {kernel} class {class}: {families} families, {per_workgroup}/workgroup, tables in LDS "
             "{tables_in_lds}, grid {grid}, LDS {lds_bytes} B ({shared_lds_bytes} shared, {family_lds_bytes} per "
             "family)".format(**r))

@@ -143,11 +143,13 @@ int whale_batch_stats(const whale_batch* b, int64_t* out12);
 int whale_batch_fallbacks(const whale_batch* b, int32_t* n);
 
 /* Launch shapes of the batched kernels for `mode`: for the forward and the
- * reverse sweep of each of the three size classes (6 rows of 8 values):
+ * reverse sweep of each of the WHALE_SIZE_CLASSES size classes (rows of 8 values,
+ * forward rows first):
  * [families, families per workgroup, slice tables in LDS (0/1), grid,
  *  LDS bytes per workgroup, of which shared by its waves, 0 (reserved),
  *  LDS bytes per family].  No reference counterpart (introspection). */
-int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* out48);
+#define WHALE_SIZE_CLASSES 6
+int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* out);
 
 /* logpdf of every family (unconditioned, src/core.jl:31-43) into
  * out_family[n_families] (may be NULL) and the conditioned batch value

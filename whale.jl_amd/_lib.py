@@ -16,6 +16,7 @@ NODE_LEAF, NODE_SPECIATION, NODE_WGD, NODE_ROOT = 0, 1, 2, 3
 COND_ROOT, COND_NONEXTINCT, COND_NONE = 0, 1, 2
 
 MODES = {"fp64": FP64_PLAIN, "fp32": FP32_SCALED, "fp64s": FP64_SCALED}
+SIZE_CLASSES = 6  # WHALE_SIZE_CLASSES
 
 _lib = None
 _synth = None

@@ -141,14 +141,15 @@ class DeviceBatch:
 
     def launches(self, mode: str = "fp32") -> list:
         """Launch shape of each batched kernel: forward and reverse sweep per size class."""
-        o = np.zeros(48, np.int64)
+        ncls = _lib.SIZE_CLASSES
+        o = np.zeros(16 * ncls, np.int64)
         check(_lib.lib().whale_batch_launches(self.h, _lib.MODES[mode], ptr(o, i64p)), "whale_batch_launches")
         keys = ["families", "per_workgroup", "tables_in_lds", "grid", "lds_bytes", "shared_lds_bytes", "reserved",
                 "family_lds_bytes"]
         out = []
         for d, name in enumerate(("forward", "backward")):
-            for k in range(3):
-                r = dict(zip(keys, (int(x) for x in o[8 * (3 * d + k):8 * (3 * d + k + 1)])))
+            for k in range(ncls):
+                r = dict(zip(keys, (int(x) for x in o[8 * (ncls * d + k):8 * (ncls * d + k + 1)])))
                 r["kernel"], r["class"] = name, k
                 out.append(r)
         return out

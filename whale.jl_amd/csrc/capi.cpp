@@ -61,6 +61,7 @@ hipError_t upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 struct whale_batch;
 namespace {
+constexpr int kTickets = 16;  // family tickets: one per class and direction, two for the FP64 pass
 int plan_batch_launches(whale_batch* b);
 
 // LDS one wave needs for family d: its largest non-root stage (items and
@@ -79,8 +80,8 @@ size_t fam_lds(const whale::FamDev& d, int mode, bool fwd, bool root_recs) {
 struct whale_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t side[2] = {};       // size classes run beside each other on these and `stream`
-  hipEvent_t fork[6] = {};
+  hipStream_t side = nullptr;  // the largest families run here beside the others
+  hipEvent_t fork[4] = {};
   int n_cu = 256;
   size_t lds_wg = 64 * 1024;   // LDS one workgroup may allocate
   size_t lds_cu = 64 * 1024;   // LDS of one compute unit
@@ -129,7 +130,7 @@ struct whale_batch {
   uint64_t n_rec = 0;
   uint32_t n_fam = 0;
   // size classes by the LDS one family needs: families order[cls_lo[c] .. cls_hi[c])
-  static constexpr int kClasses = 3;
+  static constexpr int kClasses = 6;
   uint32_t cls_lo[kClasses] = {}, cls_hi[kClasses] = {};
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
@@ -208,7 +209,7 @@ int whale_ctx_create(int device, whale_ctx** out) {
   auto c = std::make_unique<whale_ctx>();
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  for (auto& sd : c->side) HIPCHK(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   for (auto& e : c->fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -233,7 +234,7 @@ int whale_ctx_destroy(whale_ctx* c) {
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   for (auto& e : c->user) if (e) (void)hipEventDestroy(e);
   for (auto& e : c->fork) if (e) (void)hipEventDestroy(e);
-  for (auto& sd : c->side) if (sd) (void)hipStreamDestroy(sd);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return WHALE_OK;
@@ -514,16 +515,18 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   if (P.rec_ops.size() > 0xffffffffull || P.inv_ops.size() > 0xffffffffull || P.it_seg.size() > 0xffffffffull)
     return fail(WHALE_EINVAL, "batch too large");
   b->n_fam = (uint32_t)n_fam;
-  // size classes by the LDS one family needs (FP32, either direction): most
-  // families fit a small wave region and run eight to a compute unit.  Order:
-  // large class first, then small, then tiny; costliest families first.
+  // size classes by the LDS one family needs (FP32, either direction), so
+  // that each launch packs as many families per compute unit as their LDS
+  // allows.  Order: largest class first; costliest families first in a class.
   {
-    const size_t bounds[2] = {8 * 1024, 16 * 1024};
+    const size_t bounds[whale_batch::kClasses - 1] = {8 * 1024, 12 * 1024, 16 * 1024, 24 * 1024, 32 * 1024};
     std::vector<uint32_t> cls[whale_batch::kClasses];
     for (uint32_t f : P.cost_order) {
       const size_t need = std::max(fam_lds(P.fam[f], WHALE_FP32_SCALED, true, true),
                                    fam_lds(P.fam[f], WHALE_FP32_SCALED, false, true));
-      cls[need <= bounds[0] ? 0 : need <= bounds[1] ? 1 : 2].push_back(f);
+      int k = whale_batch::kClasses - 1;
+      while (k > 0 && need <= bounds[k - 1]) --k;
+      cls[k].push_back(f);
     }
     b->order_host.clear();
     for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
@@ -569,7 +572,7 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   HIPCHK(b->fam_ll.alloc(sizeof(double) * n_fam));
   AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   HIPCHK(b->acc.alloc(sizeof(double) * AL.n));
-  HIPCHK(b->ticket.alloc(sizeof(uint32_t) * 8));
+  HIPCHK(b->ticket.alloc(sizeof(uint32_t) * kTickets));
   HIPCHK(hipHostMalloc((void**)&b->pin_out, sizeof(double) * (3 + 2 * m->h.n + m->h.n_wgd), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void**)&b->pin_fam, sizeof(double) * n_fam, hipHostMallocDefault));
   // release host copies the device now owns (keep what ℓ download needs)
@@ -602,12 +605,13 @@ int whale_batch_fallbacks(const whale_batch* b, int32_t* n) {
 }
 
 int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* o) {
+  static_assert(whale_batch::kClasses == WHALE_SIZE_CLASSES, "size classes of the C ABI");
   if (!b || !o) return fail(WHALE_EINVAL, "NULL argument");
   if (mode < 0 || mode > 2) return fail(WHALE_EINVAL, "unknown mode");
   for (int dir = 0; dir < 2; ++dir)
     for (int k = 0; k < whale_batch::kClasses; ++k) {
       const LaunchCfg& C = b->cfg[mode][dir][k];
-      int64_t* r = o + 8 * (3 * dir + k);
+      int64_t* r = o + 8 * (whale_batch::kClasses * dir + k);
       r[0] = b->cls_hi[k] - b->cls_lo[k];
       r[1] = C.fpw;
       r[2] = C.tabl ? 1 : 0;
@@ -645,6 +649,7 @@ namespace {
 // whenever they fit; then the shape with the most resident waves per compute
 // unit wins (LDS-limited, at most kWavesPerCU).
 constexpr int kWavesPerCU = 8;
+constexpr int kMaxFPW = 8;
 
 bool plan_launch(const whale_model* m, bool fwd, int mode, size_t wave, uint64_t nfam, bool multi, LaunchCfg* out) {
   wave = align16(std::max<size_t>(wave, 16));
@@ -653,27 +658,28 @@ bool plan_launch(const whale_model* m, bool fwd, int mode, size_t wave, uint64_t
     return fwd ? fwd_shared_bytes(mode, tabl, m->h.tab_rows, m->h.n, m->h.n_wgd)
                : bwd_shared_bytes(mode, tabl, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd);
   };
-  struct Cand {
-    int fpw;
-    bool tabl;
-  } const cands[4] = {{8, true}, {4, true}, {1, true}, {1, false}};
-  int best = -1, best_waves = 0, best_wg = 0;
-  for (int j = 0; j < 4; ++j) {
-    if (cands[j].fpw > 1 && !multi) continue;
-    if (best >= 0 && cands[best].tabl && !cands[j].tabl) break;
-    const size_t lds = shared_of(cands[j].tabl) + (size_t)cands[j].fpw * wave;
-    if (lds > m->ctx->lds_wg) continue;
-    const int wg = (int)std::min<size_t>(m->ctx->lds_cu / lds, (size_t)(kWavesPerCU / cands[j].fpw));
-    if (wg * cands[j].fpw > best_waves) {
-      best = j;
-      best_waves = wg * cands[j].fpw;
-      best_wg = wg;
+  // tables in LDS whenever some shape allows it; then the most resident waves,
+  // then the fewest families per workgroup
+  int best_fpw = 0, best_waves = 0, best_wg = 0;
+  bool best_tabl = false;
+  for (int t = 1; t >= 0 && best_fpw == 0; --t) {
+    const bool tabl = t == 1;
+    for (int fpw = 1; fpw <= (multi ? kMaxFPW : 1); ++fpw) {
+      const size_t lds = shared_of(tabl) + (size_t)fpw * wave;
+      if (lds > m->ctx->lds_wg) break;
+      const int wg = (int)std::min<size_t>(m->ctx->lds_cu / lds, (size_t)(kWavesPerCU / fpw));
+      if (wg * fpw > best_waves) {
+        best_fpw = fpw;
+        best_waves = wg * fpw;
+        best_wg = wg;
+        best_tabl = tabl;
+      }
     }
   }
-  if (best < 0) return false;
+  if (best_fpw == 0) return false;
   LaunchCfg C;
-  C.fpw = cands[best].fpw;
-  C.tabl = cands[best].tabl;
+  C.fpw = best_fpw;
+  C.tabl = best_tabl;
   C.waves_per_cu = best_waves;
   C.L.wave_bytes = (uint32_t)wave;
   C.L.shared_bytes = (uint32_t)shared_of(C.tabl);
@@ -731,7 +737,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   const WorkRaw W = b->work();
   const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
-  HIPCHK(launch_fill_u32(st, W.ticket, 0u, 8));
+  HIPCHK(launch_fill_u32(st, W.ticket, 0u, kTickets));
   if (grad) HIPCHK(launch_fill_f64(st, W.acc, 0.0, AL.n));
   const uint32_t nf = b->n_fam;
   const bool fp32 = mode == WHALE_FP32_SCALED;
@@ -743,27 +749,32 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   BatchDev Bfb = B;
   Bfb.order = W.fb + 1;
   WorkRaw Wfb = b->work_fallback();
-  // size classes run concurrently: class 0 on the context's stream, 1 and 2 on the side streams
+  // The two largest size classes (long per-family chains, few families per
+  // compute unit) start first on the side stream; the other classes run one
+  // after the other on the context's stream beside them, largest first.
   auto batched = [&](int dir) -> int {
-    int used = 0;
-    for (int k = 0; k < whale_batch::kClasses; ++k) used += b->cls_hi[k] > b->cls_lo[k];
-    const bool fork = used > 1;
-    const int e0 = 3 * dir;
-    if (fork) HIPCHK(hipEventRecord(c->fork[e0], st));
+    const int big = whale_batch::kClasses - 2;
+    bool any_big = false, any_small = false;
+    for (int k = 0; k < whale_batch::kClasses; ++k)
+      (k >= big ? any_big : any_small) |= b->cls_hi[k] > b->cls_lo[k];
+    const bool fork = any_big && any_small;
+    if (fork) {
+      HIPCHK(hipEventRecord(c->fork[2 * dir], st));
+      HIPCHK(hipStreamWaitEvent(c->side, c->fork[2 * dir], 0));
+    }
     for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
       if (b->cls_hi[k] == b->cls_lo[k]) continue;
-      hipStream_t sk = (fork && k > 0) ? c->side[k - 1] : st;
-      if (sk != st) HIPCHK(hipStreamWaitEvent(sk, c->fork[e0], 0));
+      hipStream_t sk = (fork && k >= big) ? c->side : st;
       WorkRaw Wk = W;
-      Wk.ticket = W.ticket + 3 * dir + k;
+      Wk.ticket = W.ticket + whale_batch::kClasses * dir + k;
       const LaunchCfg& C = b->cfg[mode][dir][k];
       if (dir == 0) HIPCHK(launch_forward(mode, store_all, C, sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
       else HIPCHK(launch_backward(mode, C, sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
-      if (sk != st) HIPCHK(hipEventRecord(c->fork[e0 + k], sk));
     }
-    if (fork)
-      for (int k = 1; k < whale_batch::kClasses; ++k)
-        if (b->cls_hi[k] > b->cls_lo[k]) HIPCHK(hipStreamWaitEvent(st, c->fork[e0 + k], 0));
+    if (fork) {
+      HIPCHK(hipEventRecord(c->fork[2 * dir + 1], c->side));
+      HIPCHK(hipStreamWaitEvent(st, c->fork[2 * dir + 1], 0));
+    }
     return WHALE_OK;
   };
   auto single = [&](int dir, uint32_t f, const WorkRaw& Wf) -> int {
@@ -783,7 +794,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
       if ((rc = single(0, b->order_host[k], W)) != WHALE_OK) return rc;
   }
   if (fp32) {
-    Wfb.ticket = W.ticket + 6;
+    Wfb.ticket = W.ticket + 2 * whale_batch::kClasses;
     HIPCHK(launch_forward(WHALE_FP64_SCALED, store_all, b->cfg_fb[0], st, Bfb, M, Wfb, -1, 0, 0, W.fb));
   }
   if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
@@ -796,7 +807,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     }
     if (fp32) {
       WorkRaw Wfb2 = Wfb;
-      Wfb2.ticket = W.ticket + 7;
+      Wfb2.ticket = W.ticket + 2 * whale_batch::kClasses + 1;
       HIPCHK(launch_backward(WHALE_FP64_SCALED, b->cfg_fb[1], st, Bfb, M, Wfb2, -1, 0, 0, W.fb));
     }
   }

@@ -225,6 +225,38 @@ __device__ __forceinline__ Real wave_sum(Real v) {
   return v;
 }
 
+// Sums of a value over runs of equal keys within each 16-lane row of the wave
+// (records of one item sit on consecutive lanes), with DPP row shifts: the
+// last lane of a run then holds the run's sum (*last), so one LDS atomic per
+// run and row replaces up to 16 on the same address.  All lanes take part.
+template <int OFF>
+__device__ __forceinline__ float dpp_shr(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x110 + OFF, 0xf, 0xf, false));
+}
+template <int OFF>
+__device__ __forceinline__ double dpp_shr(double v) {
+  const long long x = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(x & 0xffffffffll), 0x110 + OFF, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), 0x110 + OFF, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int OFF, typename Real>
+__device__ __forceinline__ void row_seg_step(Real& v, uint32_t key) {
+  const Real vu = dpp_shr<OFF>(v);
+  const uint32_t ku = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x110 + OFF, 0xf, 0xf, false);
+  if (ku == key) v += vu;
+}
+template <typename Real>
+__device__ __forceinline__ Real row_seg_sum(Real v, uint32_t key, bool* last) {
+  row_seg_step<1>(v, key);
+  row_seg_step<2>(v, key);
+  row_seg_step<4>(v, key);
+  row_seg_step<8>(v, key);
+  const uint32_t kn = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x101, 0xf, 0xf, false);  // row_shl:1
+  *last = kn != key || (threadIdx.x & 15) == 15;
+  return v;
+}
+
 // atomic adds without return: LDS (ds_add) or global (FP64 atomics in L2)
 template <typename Real>
 __device__ __forceinline__ void lds_add(Real* p, Real v) {
@@ -244,8 +276,8 @@ __device__ __forceinline__ Real coef_of(Real p, int e1, int e2, int Eg) {
   return ldexp_r(p, e1 + e2 - Eg);
 }
 
-// One wavefront evaluates one family; a workgroup holds FPW such waves, each
-// on its own family, sharing the slice tables (and, in the reverse sweep, the
+// One wavefront evaluates one family; a workgroup holds blockDim.x / 64 such
+// waves (at most kMaxFPW), each on its own family, sharing the slice tables (and, in the reverse sweep, the
 // gradient accumulators) in LDS.  All synchronisation inside a family is
 // therefore wave-level.  The LDS executes a wave's instructions in order, so
 // between rows it is enough to keep the compiler from moving LDS accesses
@@ -463,8 +495,10 @@ struct FwdRoot {
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-template <typename Real, bool Scaled, bool StoreAll, int FPW, bool TabL>
-__global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+constexpr int kMaxFPW = 8;  // families (waves) per workgroup, at most
+
+template <typename Real, bool Scaled, bool StoreAll, bool TabL>
+__global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                        const uint32_t* fam_count) {
   if (fam_count) fam_end = fam_begin + *fam_count;
@@ -474,12 +508,12 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
   Real* sTab = reinterpret_cast<Real*>(smem);
   NodeC<Real> NC(smem + (TabL ? align16(2 * (size_t)M.tab_rows * sizeof(Real)) : 0), M.n);
   if (TabL) {
-    for (int r = threadIdx.x; r < M.tab_rows; r += kWG * FPW) {
+    for (int r = threadIdx.x; r < M.tab_rows; r += (int)blockDim.x) {
       sTab[2 * r] = (Real)M.phi[r];
       sTab[2 * r + 1] = (Real)M.psi[r];
     }
   }
-  NC.fill(M, threadIdx.x, kWG * FPW);
+  NC.fill(M, threadIdx.x, (int)blockDim.x);
   __syncthreads();
   unsigned char* const wb = smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
   bool first = true;
@@ -595,29 +629,42 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
           const Real ef = NC.etop[sg.c0];
           const Real eg = wgd ? Real(0) : NC.etop[sg.c1];
           const int Eg = sEi[i];
-          XSum<Real> xs;
-          for (uint32_t k = r0; k < r1; ++k) {
-            const uint32_t ops = sOps[k];
-            const uint32_t a = ops & 0xffff, b = ops >> 16;
-            const Real pk = sCoef[k];
-            const Real fa = bF[a], fb = bF[b];
-            const Real ga = wgd ? fa : bB[a], gb = wgd ? fb : bB[b];
-            const int ea = sEi[a], eb = sEi[b];
-            Real m;
-            int ex;
-            if (fa != Real(0) && gb != Real(0)) { xprod(pk * qq, fa, ea, gb, eb, &m, &ex); xs.add(m, ex); }
-            if (!wgd && ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ex); xs.add(m, ex); }
-          }
-          if (Eg == kUndef) {
+          if (Eg != kUndef) {
+            // exponent fixed: the split terms, rescaled to it, summed in FP64
+            // (wide enough for any product of the stored values)
+            double sacc = 0.0;
+            for (uint32_t k = r0; k < r1; ++k) {
+              const uint32_t ops = sOps[k];
+              const uint32_t a = ops & 0xffff, b = ops >> 16;
+              const int ea = sEi[a], eb = sEi[b];
+              if (ea == kUndef || eb == kUndef) continue;  // parts without values yet
+              const double fa = bF[a], fb = bF[b];
+              const double t = wgd ? (double)qq * fa * fb : fa * (double)bB[b] + (double)bB[a] * fb;
+              sacc += ldexp((double)sCoef[k] * t, ea + eb - Eg);
+            }
+            const Real base = wgd ? ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]) : (bF[i] * eg + bB[i] * ef);
+            v = base + (Real)sacc;
+          } else {
+            // first values of this clade: explicit exponents, then its own
+            XSum<Real> xs;
+            for (uint32_t k = r0; k < r1; ++k) {
+              const uint32_t ops = sOps[k];
+              const uint32_t a = ops & 0xffff, b = ops >> 16;
+              const Real pk = sCoef[k];
+              const Real fa = bF[a], fb = bF[b];
+              const Real ga = wgd ? fa : bB[a], gb = wgd ? fb : bB[b];
+              const int ea = sEi[a], eb = sEi[b];
+              Real m;
+              int ex;
+              if (fa != Real(0) && gb != Real(0)) { xprod(pk * qq, fa, ea, gb, eb, &m, &ex); xs.add(m, ex); }
+              if (!wgd && ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ex); xs.add(m, ex); }
+            }
             if (xs.k != kUndef) {
               int ex;
               v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
               sEi[i] = xs.k + ex + kBirth;
               sFlag[i] = kYoung;
             }
-          } else {
-            const Real base = wgd ? ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]) : (bF[i] * eg + bB[i] * ef);
-            v = base + (xs.k == kUndef ? Real(0) : ldexp_r(xs.s, xs.k - Eg));
           }
         }
         bA[i] = v;
@@ -656,9 +703,22 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
           }
         }
       };
+      // items still without values (open clades with splits): they take the
+      // explicit-exponent path until their first value
+      auto count_open = [&]() -> uint32_t {
+        uint32_t cnt = 0;
+        for (uint32_t base = 0; base < n; base += kWG) {
+          const uint32_t i = base + lane;
+          const bool open = i < n && (sFlag[i] & kPending) && sRO[i + 1] > sRO[i];
+          cnt += (uint32_t)__popcll(__ballot(open));
+        }
+        return cnt;
+      };
+      uint32_t n_open = 0;
       if (Scaled) {
         refresh();
         wave_sync();
+        n_open = count_open();
       }
       // rows 1..R-1: within_branch! (src/core.jl:83-92)
       Real* cur = bA;
@@ -667,14 +727,40 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
         bool dirty = false;
         // split sums of every item, one record per lane (lane-balanced however
         // the records are spread over the items)
-#pragma unroll 2
-        for (uint32_t k = lane; k < nrec; k += kWG) {
-          const uint32_t ops = sOps[k];
-          const uint32_t own = sOwn[k];
-          const Real t = sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
-          if (t != Real(0)) lds_add(&sAcc[own], t);
+        for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the row sums use DPP
+          const uint32_t k = base + lane;
+          uint32_t own = kNone;
+          Real t = 0;
+          if (k < nrec) {
+            const uint32_t ops = sOps[k];
+            own = sOwn[k];
+            t = sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
+          }
+          bool last;
+          t = row_seg_sum(t, own, &last);
+          if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
         }
         wave_sync();
+        // a young clade past 2^kBump is renormalised in place (its only values are
+        // this branch's rows); its exponent is raised after the row, as other items
+        // of this row still read the clade's previous row at the old scale
+        auto bump_check = [&](const ItemRow& it, uint32_t i, Real& v, uint8_t fl) {
+          if (Scaled && sizeof(Real) == sizeof(float) && (fl & kYoung) &&
+              fabsf((float)v) > (float)(1ull << kBump) && isfinite((float)v)) {
+            int ex;
+            (void)frexp_r(v, &ex);
+            const int sh = ex + kBirth;
+            v = ldexp_r(v, -sh);
+            if (StoreAll)
+              for (uint32_t q = 0; q < r; ++q) {
+                Real* p = ell + it.cell + (size_t)q * it.stride;
+                *p = ldexp_r(*p, -sh);
+              }
+            bF[i] = (Real)sh;  // bF is free after row 0
+            sFlag[i] = fl | kShifted;
+            dirty = true;
+          }
+        };
         auto update = [&](const ItemRow& it, uint32_t i) {
           Real v = cur[i];
           if (r < it.R) {
@@ -684,24 +770,7 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
             const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
             if (!Scaled || !(fl & kPending) || r1 == r0) {
               v = ph * v + ps * sAcc[i];
-              if (Scaled && sizeof(Real) == sizeof(float) && (fl & kYoung) &&
-                  fabsf((float)v) > (float)(1ull << kBump) && isfinite((float)v)) {
-                // renormalise a young clade in place (its only values are this branch's rows);
-                // its exponent is raised after the row, as other items of this row still read
-                // the clade's previous row at the old scale
-                int ex;
-                (void)frexp_r(v, &ex);
-                const int sh = ex + kBirth;
-                v = ldexp_r(v, -sh);
-                if (StoreAll)
-                  for (uint32_t q = 0; q < r; ++q) {
-                    Real* p = ell + it.cell + (size_t)q * it.stride;
-                    *p = ldexp_r(*p, -sh);
-                  }
-                bF[i] = (Real)sh;  // bF is free after row 0
-                sFlag[i] = fl | kShifted;
-                dirty = true;
-              }
+              bump_check(it, i, v, fl);
             } else {
               // clade still open: sum with explicit exponents (coefficients hold p)
               XSum<Real> xs;
@@ -728,10 +797,46 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
           nxt[i] = v;
           sAcc[i] = 0;
         };
+        if (!Scaled || n_open == 0) {
+          // no clade of the stage is still open: every cached item takes the
+          // plain update; all slots' LDS reads are issued before any is used
+          Real v0[kCache], ac[kCache], ph[kCache], ps[kCache];
+          uint8_t fl[kCache];
 #pragma unroll
-        for (int c = 0; c < kCache; ++c) {
-          const uint32_t i = lane + c * kWG;
-          if (i < n) update(cache[c], i);
+          for (int c = 0; c < kCache; ++c) {
+            if ((uint32_t)c * kWG < n) {
+              const uint32_t i = lane + c * kWG;
+              const uint32_t ii = i < n ? i : 0u;
+              const bool act = i < n && r < cache[c].R;
+              slice<Real, TabL>(M, sTab, act ? cache[c].tab + r : 0u, ph[c], ps[c]);
+              v0[c] = cur[ii];
+              ac[c] = sAcc[ii];
+              fl[c] = (Scaled && sizeof(Real) == sizeof(float)) ? sFlag[ii] : 0;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < kCache; ++c) {
+            if ((uint32_t)c * kWG < n) {
+              const uint32_t i = lane + c * kWG;
+              if (i < n) {
+                const ItemRow& it = cache[c];
+                Real v = v0[c];
+                if (r < it.R) {
+                  v = ph[c] * v + ps[c] * ac[c];
+                  bump_check(it, i, v, fl[c]);
+                  if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
+                }
+                nxt[i] = v;
+                sAcc[i] = 0;
+              }
+            }
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < kCache; ++c) {
+            const uint32_t i = lane + c * kWG;
+            if (i < n) update(cache[c], i);
+          }
         }
         for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) update(load_item(B, M, i0 + i, i, sRO), i);
         wave_sync();
@@ -745,6 +850,7 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
           wave_sync();
           refresh();
           wave_sync();
+          n_open = count_open();
         }
         Real* t = cur;
         cur = nxt;
@@ -813,28 +919,39 @@ __global__ __launch_bounds__(kWG* FPW) void k_forward(BatchDev B, ModelDev M, Wo
             v = cA * a + cB * (b + c);
           } else {
             const int Eg = sEi[i];
-            XSum<Real> xs;
-            for (uint32_t k = r0; k < r1; ++k) {
-              const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
-              const Real pk = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
-              const uint32_t x = ops & 0xffff, y = ops >> 16;
-              const int ex_ = sEi[x], ey = sEi[y];
-              Real m;
-              int ee;
-              if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); xs.add(m, ee); }
-              if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); xs.add(m, ee); }
-              if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); xs.add(m, ee); }
-            }
-            if (Eg == kUndef) {
+            if (Eg != kUndef) {
+              // exponent fixed: split terms rescaled to it, summed in FP64
+              double sacc = 0.0;
+              for (uint32_t k = r0; k < r1; ++k) {
+                const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
+                const Real pk = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
+                const uint32_t x = ops & 0xffff, y = ops >> 16;
+                const int ex_ = sEi[x], ey = sEi[y];
+                if (ex_ == kUndef || ey == kUndef) continue;
+                const double t = (double)cA * bA[x] * bA[y] + (double)cB * ((double)bF[x] * bB[y] + (double)bB[x] * bF[y]);
+                sacc += ldexp((double)pk * t, ex_ + ey - Eg);
+              }
+              const Real c = bF[i] * eg + bB[i] * ef;
+              v = cB * c + (Real)sacc;
+            } else {
+              XSum<Real> xs;
+              for (uint32_t k = r0; k < r1; ++k) {
+                const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
+                const Real pk = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
+                const uint32_t x = ops & 0xffff, y = ops >> 16;
+                const int ex_ = sEi[x], ey = sEi[y];
+                Real m;
+                int ee;
+                if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); xs.add(m, ee); }
+                if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); xs.add(m, ee); }
+                if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); xs.add(m, ee); }
+              }
               v = 0;
               if (xs.k != kUndef) {
                 int ex;
                 v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
                 sEi[i] = xs.k + ex + kBirth;  // no clade of this level is a part of another
               }
-            } else {
-              const Real c = bF[i] * eg + bB[i] * ef;
-              v = cB * c + (xs.k == kUndef ? Real(0) : ldexp_r(xs.s, xs.k - Eg));
             }
           }
           bA[i] = v;
@@ -947,8 +1064,8 @@ __device__ __forceinline__ ItemTop item_top(const SegInfo& sg, uint32_t i) {
   return t;
 }
 
-template <typename Real, bool Scaled, int FPW, bool TabL>
-__global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+template <typename Real, bool Scaled, bool TabL>
+__global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                         int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                         const uint32_t* fam_count) {
   if (fam_count) fam_end = fam_begin + *fam_count;
@@ -966,14 +1083,14 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
   double* const accX = TabL ? reinterpret_cast<double*>(smem + 2 * tab_b) - AL.eps : W.acc;
   NodeC<Real> NC(smem + (TabL ? 2 * tab_b + align16(8 * (size_t)(AL.n - AL.eps)) : 0), M.n);
   if (TabL) {
-    for (int r = threadIdx.x; r < M.tab_rows; r += kWG * FPW) {
+    for (int r = threadIdx.x; r < M.tab_rows; r += (int)blockDim.x) {
       sTab[2 * r] = (Real)M.phi[r];
       sTab[2 * r + 1] = (Real)M.psi[r];
     }
-    for (int j = threadIdx.x; j < T2; j += kWG * FPW) accPS[j] = 0;
-    for (int j = AL.eps + threadIdx.x; j < AL.n; j += kWG * FPW) accX[j] = 0.0;
+    for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) accPS[j] = 0;
+    for (int j = AL.eps + threadIdx.x; j < AL.n; j += (int)blockDim.x) accX[j] = 0.0;
   }
-  NC.fill(M, threadIdx.x, kWG * FPW);
+  NC.fill(M, threadIdx.x, (int)blockDim.x);
   __syncthreads();
   unsigned char* const wb = smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
   const Real* coefs = Scaled ? W.coef : nullptr;
@@ -1151,23 +1268,28 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
       wave_sync();
       ItemRow cache[kCache];
       ItemTop tops[kCache];
-      Real pre[kCache];  // ℓ of the next row down, loaded one row ahead
+      // ℓ of the rows below, loaded two rows ahead into two register sets
+      // that take turns (rows r and r-1 use preA and preB)
+      Real preA[kCache], preB[kCache];
       const int rtop = (int)S.r_max - 1;
 #pragma unroll
       for (int c = 0; c < kCache; ++c) {
-        pre[c] = 0;
+        preA[c] = 0;
+        preB[c] = 0;
         if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
           const uint32_t i = lane + c * kWG;
           const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
           if (i < n) {
             cache[c] = item_row(sg, i);
             tops[c] = item_top(sg, i);
-            if (rtop >= 1 && rtop < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(rtop - 1) * cache[c].stride];
+            const ItemRow& it = cache[c];
+            if (rtop >= 1 && rtop < (int)it.R) preA[c] = ell[it.cell + (size_t)(rtop - 1) * it.stride];
+            if (rtop >= 2 && rtop - 1 < (int)it.R) preB[c] = ell[it.cell + (size_t)(rtop - 2) * it.stride];
           }
         }
       }
       Real* const sT = bG;  // ψ_r·ā(r) per item, for the record pass
-      for (int r = rtop; r >= 1; --r) {
+      auto row_step = [&](const int r, Real (&pre)[kCache]) {
         // items: ℓ of row r-1, the φ part of ā(r-1) and ψ_r·ā(r)
         auto item_a = [&](const ItemRow& it, uint32_t i, Real pv) {
           const Real g = bA[i];
@@ -1195,25 +1317,31 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
           const uint32_t i = lane + c * kWG;
-          if (i < n && r >= 2 && r - 1 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(r - 2) * cache[c].stride];
+          if (i < n && r >= 3 && r - 2 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(r - 3) * cache[c].stride];
         }
         wave_sync();
         // records, one per lane: the ψ part scattered into the two parts, and
         // the split sum of the owner (forcing of d/dψ_r)
-#pragma unroll 2
-        for (uint32_t k = lane; k < nrec; k += kWG) {
-          const uint32_t ops = sOps[k];
-          const uint32_t a = ops & 0xffff, b = ops >> 16;
-          const uint32_t own = sOwn[k];
-          const Real c = sCoef[k];
-          const Real va = bV[a], vb = bV[b];
-          const Real fo = c * va * vb;
-          if (fo != Real(0)) lds_add(&sFo[own], fo);
-          const Real t = sT[own] * c;
-          if (t != Real(0)) {
-            lds_add(&bB[a], t * vb);
-            lds_add(&bB[b], t * va);
+        for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the row sums use DPP
+          const uint32_t k = base + lane;
+          uint32_t own = kNone;
+          Real fo = 0;
+          if (k < nrec) {
+            const uint32_t ops = sOps[k];
+            const uint32_t a = ops & 0xffff, b = ops >> 16;
+            own = sOwn[k];
+            const Real c = sCoef[k];
+            const Real va = bV[a], vb = bV[b];
+            fo = c * va * vb;
+            const Real t = sT[own] * c;
+            if (t != Real(0)) {
+              lds_add(&bB[a], t * vb);
+              lds_add(&bB[b], t * va);
+            }
           }
+          bool last;
+          fo = row_seg_sum(fo, own, &last);
+          if (last && own != kNone && fo != Real(0)) lds_add(&sFo[own], fo);
         }
         wave_sync();
         // d/dφ_r, d/dψ_r per segment (all lanes take part: segmented reductions)
@@ -1253,6 +1381,10 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
         Real* t = bA;
         bA = bB;
         bB = t;
+      };
+      for (int r = rtop; r >= 1; r -= 2) {
+        row_step(r, preA);
+        if (r >= 2) row_step(r - 1, preB);
       }
       if (TabL) {
         // move this stage's d/dφ_r, d/dψ_r into the global FP64 sums (each LDS
@@ -1362,11 +1494,11 @@ __global__ __launch_bounds__(kWG* FPW) void k_backward(BatchDev B, ModelDev M, W
   }
   if (TabL) {
     __syncthreads();
-    for (int j = threadIdx.x; j < T2; j += kWG * FPW) {
+    for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) {
       const Real v = accPS[j];
       if (v != Real(0)) unsafeAtomicAdd(W.acc + j, (double)v);
     }
-    for (int j = AL.eps + threadIdx.x; j < AL.n; j += kWG * FPW) {
+    for (int j = AL.eps + threadIdx.x; j < AL.n; j += (int)blockDim.x) {
       const double v = accX[j];
       if (v != 0.0) unsafeAtomicAdd(W.acc + j, v);
     }
@@ -1519,13 +1651,10 @@ static hipError_t fwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev
                               const uint32_t* fam_count) {
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
-  if (C.tabl && C.fpw == 8)
-    return go(k_forward<R, S, A, 8, true>, C.grid, 8, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
-  if (C.tabl && C.fpw == 4)
-    return go(k_forward<R, S, A, 4, true>, C.grid, 4, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  if (C.fpw < 1 || C.fpw > kMaxFPW) return hipErrorInvalidValue;
   if (C.tabl)
-    return go(k_forward<R, S, A, 1, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
-  return go(k_forward<R, S, A, 1, false>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+    return go(k_forward<R, S, A, true>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  return go(k_forward<R, S, A, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
 }
 
 template <typename R, bool S>
@@ -1534,13 +1663,10 @@ static hipError_t bwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev
                               const uint32_t* fam_count) {
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
-  if (C.tabl && C.fpw == 8)
-    return go(k_backward<R, S, 8, true>, C.grid, 8, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
-  if (C.tabl && C.fpw == 4)
-    return go(k_backward<R, S, 4, true>, C.grid, 4, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  if (C.fpw < 1 || C.fpw > kMaxFPW) return hipErrorInvalidValue;
   if (C.tabl)
-    return go(k_backward<R, S, 1, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
-  return go(k_backward<R, S, 1, false>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+    return go(k_backward<R, S, true>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+  return go(k_backward<R, S, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
 }
 
 hipError_t launch_forward(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
