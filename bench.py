@@ -35,6 +35,21 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per evaluation of `kernel` (all its size-class launches) on this
+    workload, from the FETCH_SIZE/WRITE_SIZE profile committed under profiles/
+    (copy-calibrated), or None when the profile is absent."""
+    path = os.path.join(ROOT, "profiles", "config4_fp32_gradient_hbm_traffic.md")
+    sym = {"k_backward": "`k_backward<float,true,true>`", "k_forward": "`k_forward<float,true,true,true>`"}[kernel]
+    try:
+        for line in open(path, encoding="utf-8"):
+            if line.startswith("| " + sym + " |"):
+                return float(line.strip().strip("|").split("|")[-1]) * 1e6
+    except OSError:
+        pass
+    return None
+
+
 def algorithmic_bytes(st, s_bytes, x_fwd=0, x_bwd=0):
     """Algorithmic HBM bytes of one evaluation: each ℓ cell written once and
     read once in the forward (2·s per cell), 16 B per split (two int32 ids and
@@ -220,6 +235,7 @@ def main():
         dom, ach, traffic_bytes = "k_forward", fwd_gbs, b_fwd
     log(f"roofline: fwd {fwd_gbs:.1f} GB/s ({b_fwd / 1e9:.3f} GB), bwd {bwd_gbs:.1f} GB/s "
         f"({(b_grad - b_fwd) / 1e9:.3f} GB), whole step {b_grad / (ms_step / 1e3) / 1e9:.1f} GB/s")
+    hbm = measured_traffic(dom) if args.workload == "config4" and args.families is None else None
 
     # ---- CPU baseline: the oracle (C++ restatement) on a bounded sample
     cpu = None
@@ -262,7 +278,7 @@ def main():
                                "logL+gradient, FP32 with per-clade exponents, one batched launch",
                    "families": n_fam, "cells": st["cells"], "split_terms": st["split_terms"]},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": ach / HBM_PEAK_GBS, "traffic": hbm, "kernel": dom,
                      "algorithmic_bytes": traffic_bytes},
         "cpu_baseline": cpu,
         "accuracy": {"fp32_vs_fp64_family_max_rel": rel_fam, "fp32_vs_fp64_total_rel": rel_tot,
