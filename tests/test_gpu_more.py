@@ -212,3 +212,38 @@ def test_fp32_fallback_recomputes_in_fp64(w):
     assert n_fb == 1
     assert abs(got - ref) <= 1e-9 * abs(ref)
     assert np.max(np.abs(g - g64)) <= 1e-9 * np.max(np.abs(g64))
+
+
+def test_fp32_overflowing_family_is_routed_to_fp64(w):
+    """A batch of ordinary families and the overflowing family above (first,
+    middle and last).  The first FP32 call recomputes it in FP64 after the
+    FP32 pass; later FP32 calls evaluate it in FP64 from the start, beside the
+    FP32 size classes.  Every call must give the same values and gradient."""
+    from ale_util import ale_from_tree, balanced_paralogs
+    tree, dt = "((A:0.03,B:1.0):1.0,C:1.03);", 0.005
+    lam, mu = [np.log(0.8)] * 5, [np.log(0.3)] * 5
+    mu[1] = np.log(3.0)
+    big = ale_from_tree(*balanced_paralogs(64, "A", ("B_1", "C_1")))
+    small = [ale_from_tree(*balanced_paralogs(k, "A", ("B_1", "C_1"))) for k in (1, 2, 3, 5, 8)]
+    texts = small[:2] + [big] + small[2:] + [big]
+    om = OracleModel(tree, [], dt)
+    ref = om.logpdf(om.ccds(texts), ODLWGD(lam, mu, [], 0.9), per_family=True, precision="long")
+    m = w.WhaleModel(w.DLWGD(λ=lam, μ=mu, η=0.9), w.readnw(tree), dt)
+    c = w.ccds_from_texts(texts, m)
+    db = w.device_batch(m, c)
+    v64, g64, f64 = w.gradient(m, c, mode="fp64s")
+    runs = []
+    for k in range(3):
+        v, g, fam = w.gradient(m, c, mode="fp32")
+        runs.append((v, g, fam, db.fallbacks()))
+        print("call", k, "fallbacks", runs[-1][3])
+    for v, g, fam, nfb in runs:
+        assert nfb == 2
+        assert np.max(np.abs(fam - ref) / np.abs(ref)) <= 1e-6
+        assert abs(fam[2] - f64[2]) <= 1e-12 * abs(f64[2]) and abs(fam[-1] - f64[-1]) <= 1e-12 * abs(f64[-1])
+        assert abs(v - v64) <= 1e-6 * abs(v64)
+        assert np.max(np.abs(g - g64)) <= 1e-5 * np.max(np.abs(g64))
+    # routed families never change the other modes
+    fam = w.logpdf_families(m, c, mode="fp64")
+    assert np.max(np.abs(fam - ref) / np.abs(ref)) <= 1e-12
+    assert np.array_equal(runs[1][2], runs[2][2])

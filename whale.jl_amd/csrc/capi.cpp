@@ -61,7 +61,10 @@ hipError_t upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 struct whale_batch;
 namespace {
-constexpr int kTickets = 16;  // family tickets: one per class and direction, two for the FP64 pass
+// family tickets: one per class and direction, two for the FP64 pass after an
+// FP32 pass, two for the families routed to FP64 from the start
+constexpr int kTickets = 16;
+int assign_classes(whale_batch* b);
 int plan_batch_launches(whale_batch* b);
 
 // LDS one wave needs for family d: its largest non-root stage (items and
@@ -80,8 +83,9 @@ size_t fam_lds(const whale::FamDev& d, int mode, bool fwd, bool root_recs) {
 struct whale_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;  // the largest families run here beside the others
-  hipEvent_t fork[4] = {};
+  hipStream_t side = nullptr;   // the largest families run here beside the others
+  hipStream_t side2 = nullptr;  // and families routed to FP64 here
+  hipEvent_t fork[6] = {};
   int n_cu = 256;
   size_t lds_wg = 64 * 1024;   // LDS one workgroup may allocate
   size_t lds_cu = 64 * 1024;   // LDS of one compute unit
@@ -121,7 +125,7 @@ struct whale_batch {
   whale_model* model = nullptr;
   int device = 0;
   PlanHost plan;
-  DBuf fam, stage, seg, it_seg, it_gid, it_rec, it_inv, it_cf, it_cg, it_bar, rec_ops, rec_sid, rec_own, inv_ops, inv_rec,
+  DBuf fam, stage, seg, it_seg, it_gid, it_rec, it_cf, it_cg, it_bar, rec_ops, rec_sid, rec_own,
       rec_p, clade_size, split_p, lvl, order;
   // workspace.  ell/coef/ibar hold the main pass (FP32 or FP64 values) in
   // their first 8 bytes per element and, for FP32 runs, the FP64 recomputation
@@ -135,8 +139,17 @@ struct whale_batch {
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   int last_mode = -1;
+  int last_launch = WHALE_LAUNCH_BATCHED;
   bool last_store_all = false;
-  uint32_t last_fallback = 0;  // families of the last FP32 call recomputed in FP64
+  uint32_t last_fallback = 0;  // families of the last FP32 call computed in FP64
+  // Families whose FP32 values left the range in an earlier call: FP32 calls
+  // evaluate them in FP64 with per-clade exponents from the start, beside the
+  // FP32 classes (other modes treat them like any family).  They sit last in
+  // `order`, at [route_lo, n_fam).
+  std::vector<uint8_t> routed;
+  std::vector<uint32_t> flagged_once;  // recomputed in FP64 by the last call, routed from the next
+  uint32_t route_lo = 0;
+  LaunchCfg cfg_route[3][2];
   std::vector<uint32_t> order_host;
   // captured evaluations, keyed by (mode, gradient, keep ℓ), and pinned result staging
   hipGraphExec_t graph[3][2][2] = {};
@@ -155,15 +168,12 @@ struct whale_batch {
     B.it_seg = it_seg.as<uint32_t>();
     B.it_gid = it_gid.as<uint32_t>();
     B.it_rec = it_rec.as<uint32_t>();
-    B.it_inv = it_inv.as<uint32_t>();
     B.it_cf = it_cf.as<int32_t>();
     B.it_cg = it_cg.as<int32_t>();
     B.it_bar = it_bar.as<int32_t>();
     B.rec_ops = rec_ops.as<uint32_t>();
     B.rec_sid = rec_sid.as<uint32_t>();
     B.rec_own = rec_own.as<uint16_t>();
-    B.inv_ops = inv_ops.as<uint32_t>();
-    B.inv_rec = inv_rec.as<uint32_t>();
     B.rec_p = rec_p.as<double>();
     B.clade_size = clade_size.as<uint16_t>();
     B.split_p = split_p.as<double>();
@@ -210,6 +220,7 @@ int whale_ctx_create(int device, whale_ctx** out) {
   c->device = device;
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
   for (auto& e : c->fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -235,6 +246,7 @@ int whale_ctx_destroy(whale_ctx* c) {
   for (auto& e : c->user) if (e) (void)hipEventDestroy(e);
   for (auto& e : c->fork) if (e) (void)hipEventDestroy(e);
   if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->side2) (void)hipStreamDestroy(c->side2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return WHALE_OK;
@@ -512,31 +524,12 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
                                (int)std::min(hw, 16u));
   if (!err.empty()) return fail(WHALE_EINVAL, err);
   PlanHost& P = b->plan;
-  if (P.rec_ops.size() > 0xffffffffull || P.inv_ops.size() > 0xffffffffull || P.it_seg.size() > 0xffffffffull)
+  if (P.rec_ops.size() > 0xffffffffull || P.it_seg.size() > 0xffffffffull)
     return fail(WHALE_EINVAL, "batch too large");
   b->n_fam = (uint32_t)n_fam;
-  // size classes by the LDS one family needs (FP32, either direction), so
-  // that each launch packs as many families per compute unit as their LDS
-  // allows.  Order: largest class first; costliest families first in a class.
+  b->routed.assign(n_fam, 0);
   {
-    const size_t bounds[whale_batch::kClasses - 1] = {8 * 1024, 12 * 1024, 16 * 1024, 24 * 1024, 32 * 1024};
-    std::vector<uint32_t> cls[whale_batch::kClasses];
-    for (uint32_t f : P.cost_order) {
-      const size_t need = std::max(fam_lds(P.fam[f], WHALE_FP32_SCALED, true, true),
-                                   fam_lds(P.fam[f], WHALE_FP32_SCALED, false, true));
-      int k = whale_batch::kClasses - 1;
-      while (k > 0 && need <= bounds[k - 1]) --k;
-      cls[k].push_back(f);
-    }
-    b->order_host.clear();
-    for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
-      b->cls_lo[k] = (uint32_t)b->order_host.size();
-      b->order_host.insert(b->order_host.end(), cls[k].begin(), cls[k].end());
-      b->cls_hi[k] = (uint32_t)b->order_host.size();
-    }
-  }
-  {
-    const int rc = plan_batch_launches(b.get());
+    const int rc = assign_classes(b.get());
     if (rc != WHALE_OK) return rc;
   }
   whale_ctx* c = m->ctx;
@@ -548,15 +541,12 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   HIPCHK(upload(b->it_seg, P.it_seg, st));
   HIPCHK(upload(b->it_gid, P.it_gid, st));
   HIPCHK(upload(b->it_rec, P.it_rec, st));
-  HIPCHK(upload(b->it_inv, P.it_inv, st));
   HIPCHK(upload(b->it_cf, P.it_cf, st));
   HIPCHK(upload(b->it_cg, P.it_cg, st));
   HIPCHK(upload(b->it_bar, P.it_bar, st));
   HIPCHK(upload(b->rec_ops, P.rec_ops, st));
   HIPCHK(upload(b->rec_sid, P.rec_sid, st));
   HIPCHK(upload(b->rec_own, P.rec_own, st));
-  HIPCHK(upload(b->inv_ops, P.inv_ops, st));
-  HIPCHK(upload(b->inv_rec, P.inv_rec, st));
   HIPCHK(upload(b->rec_p, P.rec_p, st));
   HIPCHK(upload(b->clade_size, P.clade_size, st));
   HIPCHK(upload(b->split_p, P.split_p, st));
@@ -577,15 +567,12 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   HIPCHK(hipHostMalloc((void**)&b->pin_fam, sizeof(double) * n_fam, hipHostMallocDefault));
   // release host copies the device now owns (keep what ℓ download needs)
   std::vector<uint32_t>().swap(P.it_rec);
-  std::vector<uint32_t>().swap(P.it_inv);
   std::vector<int32_t>().swap(P.it_cf);
   std::vector<int32_t>().swap(P.it_cg);
   std::vector<int32_t>().swap(P.it_bar);
   std::vector<uint32_t>().swap(P.rec_ops);
   std::vector<uint32_t>().swap(P.rec_sid);
   std::vector<uint16_t>().swap(P.rec_own);
-  std::vector<uint32_t>().swap(P.inv_ops);
-  std::vector<uint32_t>().swap(P.inv_rec);
   std::vector<double>().swap(P.rec_p);
   *out = b.release();
   return WHALE_OK;
@@ -726,6 +713,44 @@ int plan_batch_launches(whale_batch* b) {
   return WHALE_OK;
 }
 
+// Size classes by the LDS one family needs (FP32, either direction), so that
+// each launch packs as many families per compute unit as their LDS allows.
+// Order: largest class first, costliest families first in a class, families
+// routed to FP64 last.  Then the launch shapes.
+int assign_classes(whale_batch* b) {
+  const PlanHost& P = b->plan;
+  const size_t bounds[whale_batch::kClasses - 1] = {8 * 1024, 12 * 1024, 16 * 1024, 24 * 1024, 32 * 1024};
+  std::vector<uint32_t> cls[whale_batch::kClasses], rt;
+  for (uint32_t f : P.cost_order) {
+    if (b->routed[f]) {
+      rt.push_back(f);
+      continue;
+    }
+    const size_t need = std::max(fam_lds(P.fam[f], WHALE_FP32_SCALED, true, true),
+                                 fam_lds(P.fam[f], WHALE_FP32_SCALED, false, true));
+    int k = whale_batch::kClasses - 1;
+    while (k > 0 && need <= bounds[k - 1]) --k;
+    cls[k].push_back(f);
+  }
+  b->order_host.clear();
+  for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
+    b->cls_lo[k] = (uint32_t)b->order_host.size();
+    b->order_host.insert(b->order_host.end(), cls[k].begin(), cls[k].end());
+    b->cls_hi[k] = (uint32_t)b->order_host.size();
+  }
+  b->route_lo = (uint32_t)b->order_host.size();
+  b->order_host.insert(b->order_host.end(), rt.begin(), rt.end());
+  int rc = plan_batch_launches(b);
+  if (rc != WHALE_OK) return rc;
+  for (int mode = 0; mode < 3; ++mode)
+    for (int dir = 0; dir < 2; ++dir)
+      if (!rt.empty() &&
+          !plan_families(b, dir == 0, mode == WHALE_FP32_SCALED ? WHALE_FP64_SCALED : mode, rt.data(), rt.size(),
+                         rt.size(), true, &b->cfg_route[mode][dir]))
+        return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
+  return WHALE_OK;
+}
+
 // All device work of one evaluation on the context's stream (capturable: no
 // memset nodes, no host synchronisation).  Timing events only when `timed`.
 int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all, bool timed) {
@@ -787,6 +812,21 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     return WHALE_OK;
   };
   int rc;
+  // families routed to FP64: their whole evaluation on the second side stream
+  const bool routed = launch == WHALE_LAUNCH_BATCHED && b->route_lo < nf;
+  if (routed) {
+    HIPCHK(hipEventRecord(c->fork[4], st));
+    HIPCHK(hipStreamWaitEvent(c->side2, c->fork[4], 0));
+    const int rmode = fp32 ? WHALE_FP64_SCALED : mode;
+    WorkRaw Wr = fp32 ? b->work_fallback() : W;
+    Wr.ticket = W.ticket + 2 * whale_batch::kClasses + 2;
+    HIPCHK(launch_forward(rmode, store_all, b->cfg_route[mode][0], c->side2, B, M, Wr, -1, b->route_lo, nf));
+    if (grad) {
+      Wr.ticket = W.ticket + 2 * whale_batch::kClasses + 3;
+      HIPCHK(launch_backward(rmode, b->cfg_route[mode][1], c->side2, B, M, Wr, -1, b->route_lo, nf));
+    }
+    HIPCHK(hipEventRecord(c->fork[5], c->side2));
+  }
   if (launch == WHALE_LAUNCH_BATCHED) {
     if ((rc = batched(0)) != WHALE_OK) return rc;
   } else {
@@ -811,6 +851,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
       HIPCHK(launch_backward(WHALE_FP64_SCALED, b->cfg_fb[1], st, Bfb, M, Wfb2, -1, 0, 0, W.fb));
     }
   }
+  if (routed) HIPCHK(hipStreamWaitEvent(st, c->fork[5], 0));
   if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
   double* out = m->out.as<double>();
   HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
@@ -836,6 +877,20 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
   const bool store_all = grad || keep;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
+  if (!b->flagged_once.empty()) {
+    for (uint32_t f : b->flagged_once) b->routed[f] = 1;
+    b->flagged_once.clear();
+    const int rc = assign_classes(b);
+    if (rc != WHALE_OK) return rc;
+    HIPCHK(hipMemcpy(b->order.p, b->order_host.data(), sizeof(uint32_t) * b->n_fam, hipMemcpyHostToDevice));
+    for (auto& a : b->graph)
+      for (auto& b2 : a)
+        for (auto& g : b2)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
+  }
   if (launch == WHALE_LAUNCH_BATCHED && !c->timing) {
     // replay the captured evaluation (captured on first use)
     hipGraphExec_t& ge = b->graph[mode][grad ? 1 : 0][keep ? 1 : 0];
@@ -879,7 +934,18 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
   }
   b->last_mode = mode;
   b->last_store_all = store_all;
-  b->last_fallback = mode == WHALE_FP32_SCALED ? *reinterpret_cast<const uint32_t*>(&o[2 + 2 * n + nw]) : 0;
+  b->last_launch = launch;
+  b->last_fallback = 0;
+  if (mode == WHALE_FP32_SCALED) {
+    const uint32_t fresh = *reinterpret_cast<const uint32_t*>(&o[2 + 2 * n + nw]);
+    b->last_fallback = fresh + (launch == WHALE_LAUNCH_BATCHED ? b->n_fam - b->route_lo : 0);
+    if (fresh && launch == WHALE_LAUNCH_BATCHED) {
+      // these families will be evaluated in FP64 from the start by later FP32 calls
+      std::vector<uint32_t> list(fresh);
+      HIPCHK(hipMemcpy(list.data(), b->fb.as<uint32_t>() + 1, sizeof(uint32_t) * fresh, hipMemcpyDeviceToHost));
+      for (uint32_t f : list) b->flagged_once.push_back(f);
+    }
+  }
   return WHALE_OK;
 }
 
@@ -916,7 +982,7 @@ int whale_download_ell(whale_batch* b, int32_t f, double* out, int64_t cap, int6
   if (mode == WHALE_FP32_SCALED) {
     uint32_t flagged = 0;
     HIPCHK(hipMemcpy(&flagged, b->flag.as<uint32_t>() + f, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (flagged) {
+    if (flagged || (b->routed[f] && b->last_launch == WHALE_LAUNCH_BATCHED)) {
       mode = WHALE_FP64_SCALED;
       base = (const char*)b->work_fallback().ell;
     }

@@ -15,15 +15,12 @@ struct BatchDev {
   const uint32_t* it_seg;
   const uint32_t* it_gid;
   const uint32_t* it_rec;
-  const uint32_t* it_inv;
   const int32_t* it_cf;
   const int32_t* it_cg;
   const int32_t* it_bar;
   const uint32_t* rec_ops;
   const uint32_t* rec_sid;
   const uint16_t* rec_own;
-  const uint32_t* inv_ops;
-  const uint32_t* inv_rec;
   const double* rec_p;
   const uint16_t* clade_size;
   const double* split_p;

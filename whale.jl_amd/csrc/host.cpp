@@ -216,9 +216,9 @@ struct FamPlan {
   std::vector<StageDev> stage;      // item0 relative to family, seg0 relative
   std::vector<SegDev> seg;          // ell/ibar relative to family
   uint32_t root_seg = 0, root_item0 = 0;
-  std::vector<uint32_t> it_seg, it_gid, it_rec, it_inv;  // it_seg family-relative, rec/inv family-relative
+  std::vector<uint32_t> it_seg, it_gid, it_rec;  // it_seg, it_rec family-relative
   std::vector<int32_t> it_cf, it_cg, it_bar;
-  std::vector<uint32_t> rec_ops, rec_sid, inv_ops, inv_rec;  // inv_rec family-relative
+  std::vector<uint32_t> rec_ops, rec_sid;
   std::vector<uint16_t> rec_own;                              // stage-local item owning each record
   std::vector<uint32_t> lvl;
   uint64_t ell = 0, ibar = 0;
@@ -351,7 +351,6 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
     sd.n_items = local;
     if (local > 0xffff) { F.err = "stage with more than 65535 items"; return sd; }
     if (!is_root) F.max_items = std::max(F.max_items, local);
-    std::vector<std::vector<uint32_t>> inv_p(local), inv_s(local), inv_r(local);
     for (int e : segs) {
       const SegDev& s = F.seg[segof[e]];
       const NodeDev& nd = m.nodes[e];
@@ -390,8 +389,6 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
           F.rec_ops.push_back(l1 | (l2 << 16));
           F.rec_sid.push_back((uint32_t)(k - soff[0]));
           F.rec_own.push_back((uint16_t)me);
-          inv_p[l1].push_back(me); inv_s[l1].push_back(l2); inv_r[l1].push_back(r);
-          inv_p[l2].push_back(me); inv_s[l2].push_back(l1); inv_r[l2].push_back(r);
         }
       }
     }
@@ -399,13 +396,6 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
       uint32_t nrec = (uint32_t)F.rec_ops.size() - F.it_rec[sd.item0];
       if (is_root) F.root_recs = nrec;
       else F.max_recs = std::max(F.max_recs, nrec);
-    }
-    for (uint32_t i = 0; i < local; ++i) {
-      F.it_inv.push_back((uint32_t)F.inv_ops.size());
-      for (size_t k = 0; k < inv_p[i].size(); ++k) {
-        F.inv_ops.push_back(inv_p[i][k] | (inv_s[i][k] << 16));
-        F.inv_rec.push_back(inv_r[i][k]);
-      }
     }
     return sd;
   };
@@ -418,8 +408,6 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
   StageDev rsd = emit_stage({m.root}, true);
   if (!F.err.empty()) return;
   (void)rsd;
-  // the reorder of items per stage above appended it_inv in stage order; it_inv was appended
-  // after all items of the stage, so it_inv is aligned with items.
   // root size levels
   F.lvl.push_back(0);
   for (int c = 1; c < nc; ++c) if (csize[c] != csize[c - 1]) F.lvl.push_back((uint32_t)c);
@@ -480,7 +468,6 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
     uint32_t item_base = (uint32_t)P.it_seg.size();
     uint32_t seg_base = (uint32_t)P.seg.size();
     uint32_t rec_base = (uint32_t)P.rec_ops.size();
-    uint32_t inv_base = (uint32_t)P.inv_ops.size();
     d.n_clades = F.n_clades;
     d.n_stages = (uint32_t)F.stage.size();
     d.stage0 = (uint32_t)P.stage.size();
@@ -503,7 +490,6 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
       P.it_seg.push_back(F.it_seg[i] + seg_base);
       P.it_gid.push_back(F.it_gid[i]);
       P.it_rec.push_back(F.it_rec[i] + rec_base);
-      P.it_inv.push_back(F.it_inv[i] + inv_base);
       P.it_cf.push_back(F.it_cf[i]);
       P.it_cg.push_back(F.it_cg[i]);
       P.it_bar.push_back(F.it_bar[i]);
@@ -513,10 +499,6 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
       P.rec_sid.push_back(F.rec_sid[r]);
       P.rec_own.push_back(F.rec_own[r]);
       P.rec_p.push_back(p[d.split0 + F.rec_sid[r]]);
-    }
-    for (size_t k = 0; k < F.inv_ops.size(); ++k) {
-      P.inv_ops.push_back(F.inv_ops[k]);
-      P.inv_rec.push_back(F.inv_rec[k] + rec_base);
     }
     for (auto v : F.lvl) P.lvl.push_back(v);
     ell += F.ell;
@@ -535,9 +517,8 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
       std::swap(tmp, F);
     }
   }
-  // sentinels: it_rec/it_inv of one past the last item
+  // sentinel: it_rec of one past the last item
   P.it_rec.push_back((uint32_t)P.rec_ops.size());
-  P.it_inv.push_back((uint32_t)P.inv_ops.size());
   P.ell_cells = ell;
   P.ibar_cells = ibar;
   int64_t nclades = fam_clade_off[n_fam];

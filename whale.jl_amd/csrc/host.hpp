@@ -40,9 +40,9 @@ struct PlanHost {
   std::vector<FamDev> fam;
   std::vector<StageDev> stage;
   std::vector<SegDev> seg;
-  std::vector<uint32_t> it_seg, it_gid, it_rec, it_inv;
+  std::vector<uint32_t> it_seg, it_gid, it_rec;
   std::vector<int32_t> it_cf, it_cg, it_bar;
-  std::vector<uint32_t> rec_ops, rec_sid, inv_ops, inv_rec;
+  std::vector<uint32_t> rec_ops, rec_sid;
   std::vector<uint16_t> rec_own;   // stage-local item owning each record
   std::vector<double> rec_p;       // p of each record (the FP64 plain coefficient)
   std::vector<uint16_t> clade_size;

@@ -3,8 +3,8 @@
 //
 // The reference evaluates one family by walking the species-tree nodes in
 // wm.order and, per node, every compatible clade and every time slice
-// (src/core.jl:31-37, 83-128).  The plan regroups that work for a 64-lane
-// workgroup:
+// (src/core.jl:31-37, 83-128).  The plan regroups that work for one 64-lane
+// wavefront per family:
 //
 //   segment  a (family, node) pair whose compatible-clade set C_e is not
 //            empty; its ℓ block is R_e x |C_e| values, row-major (one row =
@@ -14,10 +14,10 @@
 //            clades form the stage's items, processed row by row in lockstep.
 //            Segments that share a clade (possible only on MUL trees) are put
 //            in different stages.
-//   item     one (segment, compatible clade) pair.  Its split list
-//            (records) and inverse split list (parent, sibling) refer to other
-//            items of the same stage by stage-local index, so the row buffers
-//            of a stage live in LDS and every operand is an LDS gather.
+//   item     one (segment, compatible clade) pair.  Its split list (records)
+//            refers to other items of the same stage by stage-local index, so
+//            the row buffers of a stage live in LDS and every operand is an
+//            LDS gather; the reverse sweep scatters through the same records.
 //   root     the last stage of every family: all clades, one row, processed
 //            in clade-size levels (the root term uses the root's own smaller
 //            clades, src/core.jl:151-158).
@@ -80,7 +80,6 @@ struct SegDev {
 //   it_seg[i]   absolute segment
 //   it_gid[i]   family-local clade id
 //   it_rec[i]   first record; records of item i are [it_rec[i], it_rec[i+1])
-//   it_inv[i]   first inverse record; [it_inv[i], it_inv[i+1])
 //   it_cf[i]    ℓ cell (relative to ell0) of child0's top row for this clade, or -1
 //   it_cg[i]    same for child1
 //   it_bar[i]   ī cell (relative to ibar0) holding d logL / d top-row value, -1 at the root
@@ -88,8 +87,5 @@ struct SegDev {
 //   rec_ops[r]  l1 | l2 << 16, stage-local items of the two parts
 //   rec_sid[r]  family-local split id (p = split_p[split0 + sid])
 //   rec_own[r]  stage-local item owning the record (records of an item are contiguous)
-// Inverse records (one per occurrence of an item as a part of a split):
-//   inv_ops[k]  parent | sibling << 16, stage-local items
-//   inv_rec[k]  absolute record of that split
 
 }  // namespace whale
