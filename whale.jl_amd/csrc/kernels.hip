@@ -257,6 +257,50 @@ __device__ __forceinline__ Real row_seg_sum(Real v, uint32_t key, bool* last) {
   return v;
 }
 
+// The same sums for records held in registers across rows: which row shifts
+// stay inside the lane's run, whether the lane ends it and how many shifts
+// the longest run of the 64 lanes needs depend on the keys only, so they are
+// found once per stage.  Each row's sum is then at most four DPP adds under
+// those lane masks, and none past what the longest run needs (most items
+// have one or two splits).
+struct RunMask {
+  bool s1, s2, s4, s8, last;
+  int steps;  // wave-uniform
+};
+__device__ __forceinline__ RunMask run_mask(uint32_t key) {
+  RunMask m;
+  m.s1 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x111, 0xf, 0xf, false) == key;
+  m.s2 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x112, 0xf, 0xf, false) == key;
+  m.s4 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x114, 0xf, 0xf, false) == key;
+  m.s8 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x118, 0xf, 0xf, false) == key;
+  const uint32_t kn = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x101, 0xf, 0xf, false);  // row_shl:1
+  m.last = key != kNone && (kn != key || (threadIdx.x & 15) == 15);
+  // a run reaching back 2^k lanes also reaches back every shorter distance
+  const bool real = key != kNone;
+  m.steps = __any(real && m.s8) ? 4 : __any(real && m.s4) ? 3 : __any(real && m.s2) ? 2 : __any(real && m.s1) ? 1 : 0;
+  return m;
+}
+template <typename Real>
+__device__ __forceinline__ Real run_sum(Real v, const RunMask& m) {
+  if (m.steps > 0) {
+    const Real u = dpp_shr<1>(v);
+    if (m.s1) v += u;
+  }
+  if (m.steps > 1) {
+    const Real u = dpp_shr<2>(v);
+    if (m.s2) v += u;
+  }
+  if (m.steps > 2) {
+    const Real u = dpp_shr<4>(v);
+    if (m.s4) v += u;
+  }
+  if (m.steps > 3) {
+    const Real u = dpp_shr<8>(v);
+    if (m.s8) v += u;
+  }
+  return v;
+}
+
 // atomic adds without return: LDS (ds_add) or global (FP64 atomics in L2)
 template <typename Real>
 __device__ __forceinline__ void lds_add(Real* p, Real v) {
@@ -316,6 +360,14 @@ __device__ __forceinline__ void slice(const ModelDev& M, const Real* sTab, uint3
 // segments are loaded once, lane s holding segment s, and items fetch theirs
 // with a lane shuffle instead of a chain of dependent global loads.  Stages of
 // more than 64 segments read them from global memory.
+// row-invariant data of one item in the row loops
+struct ItemRow {
+  uint32_t cell;    // ℓ cell of row 0, relative to the family
+  uint32_t stride;  // |C_e| of the item's segment (row pitch)
+  uint32_t tab;     // first table row of the segment's node
+  uint32_t R;       // rows of the segment
+};
+
 struct SegInfo {
   uint32_t ell, nc, R, tab, item0, ibar;
   int32_t node, kind, c0, c1, wgd;
@@ -366,6 +418,9 @@ struct SegTab {
     t.wgd = __shfl(mine.wgd, src, kWG);
     return t;
   }
+  // the row-loop data of item i of segment `sl` (items past the register
+  // cache); every lane of the wave must call this together
+  __device__ __forceinline__ ItemRow row(const BatchDev& B, const ModelDev& M, uint32_t sl, uint32_t i) const;
 };
 
 // per-node constants of the current parameters in LDS: ε at the top of each
@@ -394,25 +449,7 @@ __host__ __device__ inline size_t nodec_bytes(size_t real, int n, int nwgd) {
 // ---------------------------------------------------------------------------
 // row-invariant data of one item, cached in registers for a stage's rows
 // ---------------------------------------------------------------------------
-constexpr int kCache = 8;  // items per lane held in registers (stages of up to 512 items)
-
-struct ItemRow {
-  uint32_t cell;    // ℓ cell of row 0, relative to the family
-  uint32_t stride;  // |C_e| of the item's segment (row pitch)
-  uint32_t tab;     // first table row of the segment's node
-  uint32_t R;       // rows of the segment
-};
-
-__device__ __forceinline__ ItemRow load_item(const BatchDev& B, const ModelDev& M, uint32_t ia, uint32_t i,
-                                             const uint32_t* sRO) {
-  ItemRow it;
-  const SegDev sg = B.seg[B.it_seg[ia]];
-  it.cell = sg.ell + (i - sg.item0);
-  it.stride = sg.nc;
-  it.tab = M.node[sg.node].tab;
-  it.R = sg.R;
-  return it;
-}
+constexpr int kCache = 6;  // items per lane held in registers (stages of up to 384 items)
 
 __device__ __forceinline__ ItemRow item_row(const SegInfo& sg, uint32_t i) {
   ItemRow it;
@@ -420,6 +457,17 @@ __device__ __forceinline__ ItemRow item_row(const SegInfo& sg, uint32_t i) {
   it.stride = sg.nc;
   it.tab = sg.tab;
   it.R = sg.R;
+  return it;
+}
+
+__device__ __forceinline__ ItemRow SegTab::row(const BatchDev& B, const ModelDev& M, uint32_t sl, uint32_t i) const {
+  if (!inreg) return item_row(seg_info(B, M, seg0 + sl), i);
+  const int src = (int)sl;
+  ItemRow it;
+  it.cell = __shfl(mine.ell, src, kWG) + (i - __shfl(mine.item0, src, kWG));
+  it.stride = __shfl(mine.nc, src, kWG);
+  it.tab = __shfl(mine.tab, src, kWG);
+  it.R = __shfl(mine.R, src, kWG);
   return it;
 }
 
@@ -501,6 +549,7 @@ template <typename Real, bool Scaled, bool StoreAll, bool TabL>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                        const uint32_t* fam_count) {
+  constexpr int RC = 3;  // records per lane held in registers (192 per stage)
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x % kWG;
@@ -720,14 +769,49 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
         wave_sync();
         n_open = count_open();
       }
+      // the first RC records per lane in registers: operands, owner, coefficient
+      // (row-invariant until the coefficients are refreshed)
+      uint32_t rops[RC], rown[RC];
+      Real rcoef[RC];
+      RunMask rm[RC];
+      auto load_recs = [&]() {
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+          const uint32_t k = lane + c * kWG;
+          const bool in = k < nrec;
+          rops[c] = in ? sOps[k] : 0u;
+          rown[c] = in ? (uint32_t)sOwn[k] : kNone;
+          rcoef[c] = in ? sCoef[k] : Real(0);
+          rm[c] = run_mask(rown[c]);
+        }
+      };
+      load_recs();
       // rows 1..R-1: within_branch! (src/core.jl:83-92)
       Real* cur = bA;
       Real* nxt = bB;
       for (uint32_t r = 1; r < S.r_max; ++r) {
         bool dirty = false;
         // split sums of every item, one record per lane (lane-balanced however
-        // the records are spread over the items)
-        for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the row sums use DPP
+        // the records are spread over the items); the cached records' operand
+        // reads are all issued before any is used
+        {
+          Real va[RC], vb[RC];
+#pragma unroll
+          for (int c = 0; c < RC; ++c) {
+            if ((uint32_t)c * kWG < nrec) {
+              va[c] = cur[rops[c] & 0xffff];
+              vb[c] = cur[rops[c] >> 16];
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < RC; ++c) {
+            if ((uint32_t)c * kWG < nrec) {  // uniform: the row sums use DPP
+              const Real t = run_sum(rcoef[c] * va[c] * vb[c], rm[c]);
+              if (rm[c].last && t != Real(0)) lds_add(&sAcc[rown[c]], t);
+            }
+          }
+        }
+        for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: the row sums use DPP
           const uint32_t k = base + lane;
           uint32_t own = kNone;
           Real t = 0;
@@ -838,7 +922,11 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
             if (i < n) update(cache[c], i);
           }
         }
-        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) update(load_item(B, M, i0 + i, i, sRO), i);
+        for (uint32_t base = kCache * kWG; base < n; base += kWG) {  // uniform: lane shuffles
+          const uint32_t i = base + lane;
+          const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
+          if (i < n) update(it, i);
+        }
         wave_sync();
         if (Scaled && __any(dirty)) {
           for (uint32_t i = lane; i < n; i += kWG) {
@@ -851,6 +939,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
           refresh();
           wave_sync();
           n_open = count_open();
+          load_recs();
         }
         Real* t = cur;
         cur = nxt;
@@ -1068,6 +1157,7 @@ template <typename Real, bool Scaled, bool TabL>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                         int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                         const uint32_t* fam_count) {
+  constexpr int RC = 3;  // records per lane held in registers (192 per stage)
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x % kWG;
@@ -1289,6 +1379,19 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
       }
       Real* const sT = bG;  // ψ_r·ā(r) per item, for the record pass
+      // the first RC records per lane in registers: operands, owner, coefficient
+      uint32_t rops[RC], rown[RC];
+      Real rcoef[RC];
+      RunMask rm[RC];
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const uint32_t k = lane + c * kWG;
+        const bool in = k < nrec;
+        rops[c] = in ? sOps[k] : 0u;
+        rown[c] = in ? (uint32_t)sOwn[k] : kNone;
+        rcoef[c] = in ? sCoef[k] : Real(0);
+        rm[c] = run_mask(rown[c]);
+      }
       auto row_step = [&](const int r, Real (&pre)[kCache]) {
         // items: ℓ of row r-1, the φ part of ā(r-1) and ψ_r·ā(r)
         auto item_a = [&](const ItemRow& it, uint32_t i, Real pv) {
@@ -1310,9 +1413,10 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           const uint32_t i = lane + c * kWG;
           if (i < n) item_a(cache[c], i, pre[c]);
         }
-        for (uint32_t i = lane + kCache * kWG; i < n; i += kWG) {
-          const ItemRow it = load_item(B, M, i0 + i, i, sRO);
-          item_a(it, i, r < (int)it.R ? ell[it.cell + (size_t)(r - 1) * it.stride] : Real(0));
+        for (uint32_t base = kCache * kWG; base < n; base += kWG) {  // uniform: lane shuffles
+          const uint32_t i = base + lane;
+          const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
+          if (i < n) item_a(it, i, r < (int)it.R ? ell[it.cell + (size_t)(r - 1) * it.stride] : Real(0));
         }
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
@@ -1321,8 +1425,32 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
         wave_sync();
         // records, one per lane: the ψ part scattered into the two parts, and
-        // the split sum of the owner (forcing of d/dψ_r)
-        for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the row sums use DPP
+        // the split sum of the owner (forcing of d/dψ_r).  The cached records'
+        // reads are all issued before any is used.
+        {
+          Real va[RC], vb[RC], g[RC];
+#pragma unroll
+          for (int c = 0; c < RC; ++c) {
+            if ((uint32_t)c * kWG < nrec) {
+              va[c] = bV[rops[c] & 0xffff];
+              vb[c] = bV[rops[c] >> 16];
+              g[c] = rown[c] != kNone ? sT[rown[c]] : Real(0);
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < RC; ++c) {
+            if ((uint32_t)c * kWG < nrec) {  // uniform: the row sums use DPP
+              const Real t = g[c] * rcoef[c];
+              if (t != Real(0)) {
+                lds_add(&bB[rops[c] & 0xffff], t * vb[c]);
+                lds_add(&bB[rops[c] >> 16], t * va[c]);
+              }
+              const Real fo = run_sum(rcoef[c] * va[c] * vb[c], rm[c]);
+              if (rm[c].last && fo != Real(0)) lds_add(&sFo[rown[c]], fo);
+            }
+          }
+        }
+        for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: the row sums use DPP
           const uint32_t k = base + lane;
           uint32_t own = kNone;
           Real fo = 0;
@@ -1375,7 +1503,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
         for (uint32_t base = kCache * kWG; base < n; base += kWG) {
           const uint32_t i = base + lane;
-          item_c(i < n ? load_item(B, M, i0 + i, i, sRO) : cache[0], i, i < n);
+          item_c(ST.row(B, M, i < n ? sSegL[i] : 0u, i), i, i < n);
         }
         wave_sync();
         Real* t = bA;
