@@ -83,9 +83,10 @@ size_t fam_lds(const whale::FamDev& d, int mode, bool fwd, bool root_recs) {
 struct whale_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;   // the largest families run here beside the others
-  hipStream_t side2 = nullptr;  // and families routed to FP64 here
-  hipEvent_t fork[6] = {};
+  hipStream_t side = nullptr;   // size classes run here and on side3 beside the context's stream
+  hipStream_t side3 = nullptr;
+  hipStream_t side2 = nullptr;  // families routed to FP64 run here
+  hipEvent_t fork[8] = {};
   int n_cu = 256;
   size_t lds_wg = 64 * 1024;   // LDS one workgroup may allocate
   size_t lds_cu = 64 * 1024;   // LDS of one compute unit
@@ -221,6 +222,7 @@ int whale_ctx_create(int device, whale_ctx** out) {
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->side3, hipStreamNonBlocking));
   for (auto& e : c->fork) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, device));
@@ -247,6 +249,7 @@ int whale_ctx_destroy(whale_ctx* c) {
   for (auto& e : c->fork) if (e) (void)hipEventDestroy(e);
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->side2) (void)hipStreamDestroy(c->side2);
+  if (c->side3) (void)hipStreamDestroy(c->side3);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return WHALE_OK;
@@ -774,32 +777,38 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   BatchDev Bfb = B;
   Bfb.order = W.fb + 1;
   WorkRaw Wfb = b->work_fallback();
-  // The two largest size classes (long per-family chains, few families per
-  // compute unit) start first on the side stream; the other classes run one
-  // after the other on the context's stream beside them, largest first.
+  // The size classes run on three streams at once (the context's stream and
+  // two side streams), each taking classes largest first: a class's launch
+  // holds a compute unit's LDS until its slowest family ends, and the classes
+  // beside it fill what the others leave free.
   auto batched = [&](int dir) -> int {
-    const int big = whale_batch::kClasses - 2;
-    bool any_big = false, any_small = false;
+    constexpr int kLanes = 3;
+    hipStream_t lanes[kLanes] = {st, c->side, c->side3};
+    // classes in launch order and their streams
+    static const int kOrder[whale_batch::kClasses] = {5, 4, 1, 3, 2, 0};
+    static const int kLane[whale_batch::kClasses] = {2, 2, 1, 0, 1, 0};  // by class
+    auto lane_of = [](int k) { return kLane[k]; };
+    bool used[kLanes] = {};
     for (int k = 0; k < whale_batch::kClasses; ++k)
-      (k >= big ? any_big : any_small) |= b->cls_hi[k] > b->cls_lo[k];
-    const bool fork = any_big && any_small;
-    if (fork) {
-      HIPCHK(hipEventRecord(c->fork[2 * dir], st));
-      HIPCHK(hipStreamWaitEvent(c->side, c->fork[2 * dir], 0));
-    }
-    for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
+      if (b->cls_hi[k] > b->cls_lo[k]) used[lane_of(k)] = true;
+    HIPCHK(hipEventRecord(c->fork[3 * dir], st));
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], c->fork[3 * dir], 0));
+    for (int j = 0; j < whale_batch::kClasses; ++j) {
+      const int k = kOrder[j];
       if (b->cls_hi[k] == b->cls_lo[k]) continue;
-      hipStream_t sk = (fork && k >= big) ? c->side : st;
+      hipStream_t sk = lanes[lane_of(k)];
       WorkRaw Wk = W;
       Wk.ticket = W.ticket + whale_batch::kClasses * dir + k;
       const LaunchCfg& C = b->cfg[mode][dir][k];
       if (dir == 0) HIPCHK(launch_forward(mode, store_all, C, sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
       else HIPCHK(launch_backward(mode, C, sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
     }
-    if (fork) {
-      HIPCHK(hipEventRecord(c->fork[2 * dir + 1], c->side));
-      HIPCHK(hipStreamWaitEvent(st, c->fork[2 * dir + 1], 0));
-    }
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) {
+        HIPCHK(hipEventRecord(c->fork[3 * dir + l], lanes[l]));
+        HIPCHK(hipStreamWaitEvent(st, c->fork[3 * dir + l], 0));
+      }
     return WHALE_OK;
   };
   auto single = [&](int dir, uint32_t f, const WorkRaw& Wf) -> int {
@@ -815,8 +824,8 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   // families routed to FP64: their whole evaluation on the second side stream
   const bool routed = launch == WHALE_LAUNCH_BATCHED && b->route_lo < nf;
   if (routed) {
-    HIPCHK(hipEventRecord(c->fork[4], st));
-    HIPCHK(hipStreamWaitEvent(c->side2, c->fork[4], 0));
+    HIPCHK(hipEventRecord(c->fork[6], st));
+    HIPCHK(hipStreamWaitEvent(c->side2, c->fork[6], 0));
     const int rmode = fp32 ? WHALE_FP64_SCALED : mode;
     WorkRaw Wr = fp32 ? b->work_fallback() : W;
     Wr.ticket = W.ticket + 2 * whale_batch::kClasses + 2;
@@ -825,7 +834,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
       Wr.ticket = W.ticket + 2 * whale_batch::kClasses + 3;
       HIPCHK(launch_backward(rmode, b->cfg_route[mode][1], c->side2, B, M, Wr, -1, b->route_lo, nf));
     }
-    HIPCHK(hipEventRecord(c->fork[5], c->side2));
+    HIPCHK(hipEventRecord(c->fork[7], c->side2));
   }
   if (launch == WHALE_LAUNCH_BATCHED) {
     if ((rc = batched(0)) != WHALE_OK) return rc;
@@ -851,7 +860,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
       HIPCHK(launch_backward(WHALE_FP64_SCALED, b->cfg_fb[1], st, Bfb, M, Wfb2, -1, 0, 0, W.fb));
     }
   }
-  if (routed) HIPCHK(hipStreamWaitEvent(st, c->fork[5], 0));
+  if (routed) HIPCHK(hipStreamWaitEvent(st, c->fork[7], 0));
   if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
   double* out = m->out.as<double>();
   HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
