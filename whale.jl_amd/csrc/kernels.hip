@@ -4,8 +4,9 @@
 //                  src/model.jl:162-191; getα/_ϵ/_ϕ/_ψ, src/bdputil.jl:6-11)
 //                  plus the per-row partial derivatives the reverse sweep needs
 //   k_forward      the DP of every family (whale!/whalewgd!/within_branch!/
-//                  whaleroot!, src/core.jl:83-158), one family per workgroup,
-//                  stage by stage, rows in lockstep, row buffers in LDS
+//                  whaleroot!, src/core.jl:83-158), one family per wavefront
+//                  (several per workgroup), stage by stage, the items of a
+//                  stage in lockstep row by row, row buffers in LDS
 //   k_backward     reverse sweep of the same DAG: d logL / d ℓ, accumulated
 //                  into d logL / d(φ, ψ, ε, q, A, B) of the slice tables
 //   k_tail         table adjoint (reverse of the ε recursion, chain rule to
