@@ -75,24 +75,25 @@ __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~(siz
 // on the host and carve the buffers on the device.
 //   forward stage:  bA bB bF sAcc [n] | sCoef sP [nr] | sOps [nr] | sOwn [nr] (u16) |
 //                   sRO [n+1] | sSegL [n] (u16) | sEi [n] | sFlag [n] (scaled forms)
-//   forward root:   bA bF bB [n] | sCoef [nr] | sOps [nr] | sRO [n+1] | sEi [n] (scaled)
+//   forward root:   bA bF bB [n] | sCoef [nr] | sOps [nr] | sOwn [nr] (u16) | sRO [n+1] | sEi [n] (scaled)
 //   reverse stage:  bA bB bV bG sFo [n] | sCoef [nr] | sOps [nr] | sOwn [nr] (u16) |
 //                   sRO [n+1] | sSegL [n] (u16)
-//   reverse root:   bA bB bV bG [n] | sCoef [nr] | sOps [nr] | sRO [n+1]
+//   reverse root:   bA bB bV bG [n] | sCoef [nr] | sOps [nr] | sOwn [nr] (u16) | sRO [n+1]
 // (root records are staged only when they fit: nr = 0 reads them from global memory)
 __host__ __device__ inline size_t fwd_stage_bytes(size_t s, bool scaled, size_t n, size_t nr) {
   return align16(4 * n * s) + align16(2 * nr * s) + align16(4 * nr) + align16(2 * nr) + align16(4 * (n + 1)) +
          align16(2 * n) + (scaled ? align16(4 * n) + align16(n) : 0);
 }
 __host__ __device__ inline size_t fwd_root_bytes(size_t s, bool scaled, size_t n, size_t nr) {
-  return align16(3 * n * s) + align16(nr * s) + align16(4 * nr) + align16(4 * (n + 1)) + (scaled ? align16(4 * n) : 0);
+  return align16(3 * n * s) + align16(nr * s) + align16(4 * nr) + align16(2 * nr) + align16(4 * (n + 1)) +
+         (scaled ? align16(4 * n) : 0);
 }
 __host__ __device__ inline size_t bwd_stage_bytes(size_t s, size_t n, size_t nr) {
   return align16(5 * n * s) + align16(nr * s) + align16(4 * nr) + align16(2 * nr) + align16(4 * (n + 1)) +
          align16(2 * n);
 }
 __host__ __device__ inline size_t bwd_root_bytes(size_t s, size_t n, size_t nr) {
-  return align16(4 * n * s) + align16(nr * s) + align16(4 * nr) + align16(4 * (n + 1));
+  return align16(4 * n * s) + align16(nr * s) + align16(4 * nr) + align16(2 * nr) + align16(4 * (n + 1));
 }
 
 // LDS of one launch of k_forward / k_backward: every wave of a workgroup runs

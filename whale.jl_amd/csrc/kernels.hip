@@ -524,6 +524,7 @@ template <typename Real>
 struct FwdRoot {
   Real *bA, *bF, *bB, *sCoef;
   uint32_t *sOps, *sRO;
+  uint16_t* sOwn;
   int32_t* sEi;
   __device__ FwdRoot(unsigned char* p, size_t n, size_t nr) {
     const size_t s = sizeof(Real);
@@ -535,6 +536,8 @@ struct FwdRoot {
     p += align16(nr * s);
     sOps = reinterpret_cast<uint32_t*>(p);
     p += align16(4 * nr);
+    sOwn = reinterpret_cast<uint16_t*>(p);
+    p += align16(2 * nr);
     sRO = reinterpret_cast<uint32_t*>(p);
     p += align16(4 * (n + 1));
     sEi = reinterpret_cast<int32_t*>(p);
@@ -742,15 +745,13 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
       // scaled forms: coefficients for the exponents now fixed.  A record whose
       // clade is still open keeps p (the open clade sums explicit exponents).
       auto refresh = [&]() {
-        for (uint32_t i = lane; i < n; i += kWG) {
-          const int Eg = sEi[i];
-          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-            const uint32_t ops = sOps[k];
-            const Real p = sP[k];
-            const Real c = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
-            sCoef[k] = Eg == kUndef ? p : c;
-            if (StoreAll) W.coef[rb + k] = c;
-          }
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          const uint32_t ops = sOps[k];
+          const int Eg = sEi[sOwn[k]];
+          const Real p = sP[k];
+          const Real c = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
+          sCoef[k] = Eg == kUndef ? p : c;
+          if (StoreAll) W.coef[rb + k] = c;
         }
       };
       // items still without values (open clades with splits): they take the
@@ -972,6 +973,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
       Real* const bB = T.bB;
       Real* const sCoef = T.sCoef;
       uint32_t* const sOps = T.sOps;
+      uint16_t* const sOwn = T.sOwn;
       uint32_t* const sRO = T.sRO;
       int32_t* const sEi = T.sEi;
       for (uint32_t i = lane; i < n; i += kWG) {
@@ -986,6 +988,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
 #pragma unroll 4
         for (uint32_t k = lane; k < nrec; k += kWG) {
           sOps[k] = B.rec_ops[rb + k];
+          sOwn[k] = B.rec_own[rb + k];
           sCoef[k] = (Real)B.rec_p[rb + k];
         }
       }
@@ -1070,13 +1073,11 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
       }
       if (Scaled && StoreAll) {
         // final coefficients of the root records, for the reverse sweep
-        for (uint32_t i = lane; i < n; i += kWG) {
-          const int Eg = sEi[i];
-          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-            const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
-            const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
-            W.coef[rb + k] = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
-          }
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
+          const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
+          const int Eg = sEi[inl ? sOwn[k] : B.rec_own[rb + k]];
+          W.coef[rb + k] = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
         }
       }
     }
@@ -1119,6 +1120,7 @@ template <typename Real>
 struct BwdRoot {
   Real *bA, *bB, *bV, *bG, *sCoef;
   uint32_t *sOps, *sRO;
+  uint16_t* sOwn;
   __device__ BwdRoot(unsigned char* p, size_t n, size_t nr) {
     const size_t s = sizeof(Real);
     bA = reinterpret_cast<Real*>(p);
@@ -1130,6 +1132,8 @@ struct BwdRoot {
     p += align16(nr * s);
     sOps = reinterpret_cast<uint32_t*>(p);
     p += align16(4 * nr);
+    sOwn = reinterpret_cast<uint16_t*>(p);
+    p += align16(2 * nr);
     sRO = reinterpret_cast<uint32_t*>(p);
   }
 };
@@ -1225,6 +1229,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
       Real* const bG = T.bG;
       Real* const sCoef = T.sCoef;
       uint32_t* const sOps = T.sOps;
+      uint16_t* const sOwn = T.sOwn;
       uint32_t* const sRO = T.sRO;
       const LevelBounds LB(B, F, lane);
       for (uint32_t i = lane; i < n; i += kWG) {
@@ -1239,50 +1244,50 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
       if (inl)
         for (uint32_t k = lane; k < nrec; k += kWG) {
           sOps[k] = B.rec_ops[rb + k];
+          sOwn[k] = B.rec_own[rb + k];
           sCoef[k] = coef_at(rb + k);
         }
       wave_sync();
-      auto rec = [&](uint32_t k, uint32_t& ops, Real& c) {
-        if (inl) { ops = sOps[k]; c = sCoef[k]; }
-        else { ops = B.rec_ops[rb + k]; c = coef_at(rb + k); }
+      // record k: operands, coefficient, owning clade
+      auto rec = [&](uint32_t k, uint32_t& ops, Real& c, uint32_t& own) {
+        if (inl) { ops = sOps[k]; c = sCoef[k]; own = sOwn[k]; }
+        else { ops = B.rec_ops[rb + k]; c = coef_at(rb + k); own = B.rec_own[rb + k]; }
       };
       const Real Lr = bV[n - 1];
       const bool ok = Lr > Real(0) && isfinite((double)Lr);
       if (ok) {
         if (lane == 0) bA[n - 1] = Real(1) / Lr;
         wave_sync();
-        // d logL / d ℓ of each root clade, larger clades first, scattered into the parts
+        // d logL / d ℓ of each root clade, larger clades first, scattered into
+        // the parts; a level's records (those of its clades) one per lane
         for (int Lv = (int)F.n_lvl - 1; Lv >= 0; --Lv) {
           const uint32_t a0 = LB.at(B, F, (uint32_t)Lv), a1 = LB.at(B, F, (uint32_t)Lv + 1);
-          for (uint32_t i = a0 + lane; i < a1; i += kWG) {
-            const Real t = cA * bA[i];
+          const uint32_t k1 = sRO[a1];
+          for (uint32_t k = sRO[a0] + lane; k < k1; k += kWG) {
+            uint32_t ops, own;
+            Real c;
+            rec(k, ops, c, own);
+            const Real t = cA * bA[own] * c;
             if (t == Real(0)) continue;
-            for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-              uint32_t ops;
-              Real c;
-              rec(k, ops, c);
-              const uint32_t x = ops & 0xffff, y = ops >> 16;
-              lds_add(&bA[x], t * c * bV[y]);
-              lds_add(&bA[y], t * c * bV[x]);
-            }
+            const uint32_t x = ops & 0xffff, y = ops >> 16;
+            lds_add(&bA[x], t * bV[y]);
+            lds_add(&bA[y], t * bV[x]);
           }
           wave_sync();
         }
         Real sA = 0, sBc = 0, sF = 0, sG = 0;
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          uint32_t ops, own;
+          Real c;
+          rec(k, ops, c, own);
+          const Real g = bA[own];
+          const uint32_t x = ops & 0xffff, y = ops >> 16;
+          sA += g * (c * bV[x] * bV[y]);
+          sBc += g * (c * (bB[x] * bG[y] + bG[x] * bB[y]));
+        }
         for (uint32_t i = lane; i < n; i += kWG) {
           const Real g = bA[i];
-          Real a = 0, b = 0;
-          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-            uint32_t ops;
-            Real c;
-            rec(k, ops, c);
-            const uint32_t x = ops & 0xffff, y = ops >> 16;
-            a += c * bV[x] * bV[y];
-            b += c * (bB[x] * bG[y] + bG[x] * bB[y]);
-          }
-          const Real cc = bB[i] * eg + bG[i] * ef;
-          sA += g * a;
-          sBc += g * (b + cc);
+          sBc += g * (bB[i] * eg + bG[i] * ef);
           sF += g * bB[i];
           sG += g * bG[i];
         }
@@ -1303,17 +1308,15 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           const Real eo = slot == 0 ? eg : ef;
           for (uint32_t i = lane; i < n; i += kWG) bV[i] = bA[i] * eo;
           wave_sync();
-          for (uint32_t i = lane; i < n; i += kWG) {
-            const Real g = bA[i];
+          for (uint32_t k = lane; k < nrec; k += kWG) {
+            uint32_t ops, own;
+            Real c;
+            rec(k, ops, c, own);
+            const Real g = bA[own] * c;
             if (g == Real(0)) continue;
-            for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-              uint32_t ops;
-              Real c;
-              rec(k, ops, c);
-              const uint32_t x = ops & 0xffff, y = ops >> 16;
-              lds_add(&bV[x], g * c * other[y]);
-              lds_add(&bV[y], g * c * other[x]);
-            }
+            const uint32_t x = ops & 0xffff, y = ops >> 16;
+            lds_add(&bV[x], g * other[y]);
+            lds_add(&bV[y], g * other[x]);
           }
           wave_sync();
           for (uint32_t i = lane; i < n; i += kWG) ibar[sg.ibar + slot * n + i] = cB * bV[i];
@@ -1539,12 +1542,6 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
       }
       // row 0: adjoints of the child tops, ε and q
-      for (uint32_t i = lane; i < n; i += kWG) {
-        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
-        bV[i] = cf >= 0 ? ell[cf] : Real(0);
-        bG[i] = cg >= 0 ? ell[cg] : Real(0);
-      }
-      wave_sync();
       // pass over all items: cached tops for the first kCache per lane
       auto each = [&](auto&& fn) {
 #pragma unroll
@@ -1558,62 +1555,80 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           if (i < n) fn(item_top(sg, i), i);
         }
       };
-      // ε̄, q̄ and the direct terms of the child0 slot
+      // pass over all records: the first RC per lane from registers
+      auto each_rec = [&](auto&& fn) {
+#pragma unroll
+        for (int c = 0; c < RC; ++c)
+          if ((uint32_t)c * kWG < nrec && rown[c] != kNone) fn(rops[c], rown[c], rcoef[c]);
+        for (uint32_t k = RC * kWG + lane; k < nrec; k += kWG) fn(sOps[k], (uint32_t)sOwn[k], sCoef[k]);
+      };
+      // child tops; a WGD item's one child also stands in the child1 slot, so
+      // the split terms below read the "other" top from bG for both kinds
+      bool any_wgd = false;
+      each([&](const ItemTop& tp, uint32_t i) {
+        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
+        const Real v0 = cf >= 0 ? ell[cf] : Real(0);
+        bV[i] = v0;
+        bG[i] = tp.kind == WHALE_NODE_WGD ? v0 : (cg >= 0 ? ell[cg] : Real(0));
+        sFo[i] = 0;
+        any_wgd |= tp.kind == WHALE_NODE_WGD;
+      });
+      wave_sync();
+      if (__any(any_wgd)) {
+        // WGD retention sums Σ c·ℓ(γ1)ℓ(γ2) over the child's tops, per owner
+        each_rec([&](uint32_t ops, uint32_t own, Real c) {
+          const Real t = c * bV[ops & 0xffff] * bV[ops >> 16];
+          if (t != Real(0)) lds_add(&sFo[own], t);
+        });
+        wave_sync();
+      }
+      // ε̄, q̄ and the direct terms of the child0 slot; then each owner's
+      // factor of its split terms in that slot
       each([&](const ItemTop& tp, uint32_t i) {
         const Real g = bA[i];
+        Real fac = 0;
         if (tp.kind == WHALE_NODE_SPECIATION) {
           const Real eg = NC.etop[tp.c1];
           acc_add<TabL>(accX + AL.eps + tp.c1, (double)(g * bV[i]));
           acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * bG[i]));
           bB[i] = g * eg;
+          fac = g;
         } else if (tp.kind == WHALE_NODE_WGD) {
           const Real qq = NC.q[tp.wgd];
           const Real ef = NC.etop[tp.c0];
-          Real ret = 0;
-          for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-            const uint32_t ops = sOps[k];
-            ret += sCoef[k] * bV[ops & 0xffff] * bV[ops >> 16];
-          }
+          const Real ret = sFo[i];
           acc_add<TabL>(accX + AL.q + tp.wgd, (double)(g * (ret - bV[i] + Real(2) * ef * bV[i])));
           acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * Real(2) * qq * bV[i]));
           bB[i] = g * ((Real(1) - qq) + Real(2) * qq * ef);
+          fac = qq * g;
         }
+        sFo[i] = fac;
       });
       wave_sync();
       // split terms of the child0 slot: speciation pairs child0 parts with child1
       // tops of the other part; a WGD pairs both parts on its one child
-      each([&](const ItemTop& tp, uint32_t i) {
-        if (tp.kind != WHALE_NODE_SPECIATION && tp.kind != WHALE_NODE_WGD) return;
-        const bool wgd = tp.kind == WHALE_NODE_WGD;
-        const Real g = wgd ? NC.q[tp.wgd] * bA[i] : bA[i];
+      each_rec([&](uint32_t ops, uint32_t own, Real c) {
+        const Real g = sFo[own] * c;
         if (g == Real(0)) return;
-        const Real* other = wgd ? bV : bG;
-        for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-          const uint32_t ops = sOps[k];
-          const uint32_t a = ops & 0xffff, b = ops >> 16;
-          const Real c = sCoef[k];
-          lds_add(&bB[a], g * c * other[b]);
-          lds_add(&bB[b], g * c * other[a]);
-        }
+        const uint32_t a = ops & 0xffff, b = ops >> 16;
+        lds_add(&bB[a], g * bG[b]);
+        lds_add(&bB[b], g * bG[a]);
       });
       wave_sync();
       each([&](const ItemTop& tp, uint32_t i) {
         if (tp.kind == WHALE_NODE_SPECIATION || tp.kind == WHALE_NODE_WGD) ibar[tp.ib] = bB[i];
-        if (tp.kind == WHALE_NODE_SPECIATION) bB[i] = bA[i] * NC.etop[tp.c0];
+        const bool sp = tp.kind == WHALE_NODE_SPECIATION;
+        if (sp) bB[i] = bA[i] * NC.etop[tp.c0];
+        sFo[i] = sp ? bA[i] : Real(0);
       });
       wave_sync();
       // child1 slot of speciation nodes
-      each([&](const ItemTop& tp, uint32_t i) {
-        if (tp.kind != WHALE_NODE_SPECIATION) return;
-        const Real g = bA[i];
+      each_rec([&](uint32_t ops, uint32_t own, Real c) {
+        const Real g = sFo[own] * c;
         if (g == Real(0)) return;
-        for (uint32_t k = sRO[i]; k < sRO[i + 1]; ++k) {
-          const uint32_t ops = sOps[k];
-          const uint32_t a = ops & 0xffff, b = ops >> 16;
-          const Real c = sCoef[k];
-          lds_add(&bB[a], g * c * bV[b]);
-          lds_add(&bB[b], g * c * bV[a]);
-        }
+        const uint32_t a = ops & 0xffff, b = ops >> 16;
+        lds_add(&bB[a], g * bV[b]);
+        lds_add(&bB[b], g * bV[a]);
       });
       wave_sync();
       each([&](const ItemTop& tp, uint32_t i) {
