@@ -200,25 +200,6 @@ __global__ void k_tables(ModelDev M, double* eps, double* phi, double* psi, doub
 // ---------------------------------------------------------------------------
 // wave-level reductions
 // ---------------------------------------------------------------------------
-// Segmented inclusive sums of two values over runs of equal keys on
-// consecutive lanes; the last lane of a run holds the run's sums (*tail).
-template <typename Real>
-__device__ __forceinline__ void seg_scan2(Real& v1, Real& v2, uint32_t key, bool* tail) {
-  const int lane = threadIdx.x & (kWG - 1);
-#pragma unroll
-  for (int off = 1; off < kWG; off <<= 1) {
-    const Real u1 = __shfl_up(v1, off, kWG);
-    const Real u2 = __shfl_up(v2, off, kWG);
-    const uint32_t ku = __shfl_up(key, off, kWG);
-    if (lane >= off && ku == key) {
-      v1 += u1;
-      v2 += u2;
-    }
-  }
-  const uint32_t kn = __shfl_down(key, 1, kWG);
-  *tail = (lane == kWG - 1) || (kn != key);
-}
-
 template <typename Real>
 __device__ __forceinline__ Real wave_sum(Real v) {
 #pragma unroll
@@ -1382,6 +1363,15 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           }
         }
       }
+      // runs of items sharing a segment (table row), for the d/dφ, d/dψ sums
+      RunMask im[kCache];
+#pragma unroll
+      for (int c = 0; c < kCache; ++c) {
+        if ((uint32_t)c * kWG < n) {
+          const uint32_t i = lane + c * kWG;
+          im[c] = run_mask(i < n ? cache[c].tab : kNone);
+        }
+      }
       Real* const sT = bG;  // ψ_r·ā(r) per item, for the record pass
       // the first RC records per lane in registers: operands, owner, coefficient
       uint32_t rops[RC], rown[RC];
@@ -1477,18 +1467,20 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
         wave_sync();
         // d/dφ_r, d/dψ_r per segment (all lanes take part: segmented reductions)
-        auto item_c = [&](const ItemRow& it, uint32_t i, bool valid) {
+        // (the items of one segment share its table row: sums over each
+        // 16-lane run of them, one LDS atomic per run)
+        auto item_c = [&](const ItemRow& it, uint32_t i, bool valid, const RunMask& rmk) {
           Real cphi = 0, cpsi = 0;
-          uint32_t key = kNone;
-          if (valid && r < (int)it.R) {
+          const bool on = valid && r < (int)it.R;
+          if (on) {
             const Real g = bA[i];
             cphi = g * bV[i];
             cpsi = g * sFo[i];
-            key = it.tab;
           }
-          bool tail;
-          seg_scan2(cphi, cpsi, key, &tail);
-          if (tail && key != kNone) {
+          cphi = run_sum(cphi, rmk);
+          cpsi = run_sum(cpsi, rmk);
+          if (rmk.last && on) {
+            const uint32_t key = it.tab;
             if (TabL) {
               if (cphi != Real(0)) lds_add(&accPS[AL.phi + key + r], cphi);
               if (cpsi != Real(0)) lds_add(&accPS[AL.psi + key + r], cpsi);
@@ -1500,14 +1492,15 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         };
 #pragma unroll
         for (int c = 0; c < kCache; ++c) {
-          if ((uint32_t)c * kWG < n) {
+          if ((uint32_t)c * kWG < n) {  // uniform: the run sums use DPP
             const uint32_t i = lane + c * kWG;
-            item_c(cache[c], i, i < n);
+            item_c(cache[c], i, i < n, im[c]);
           }
         }
         for (uint32_t base = kCache * kWG; base < n; base += kWG) {
           const uint32_t i = base + lane;
-          item_c(ST.row(B, M, i < n ? sSegL[i] : 0u, i), i, i < n);
+          const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
+          item_c(it, i, i < n, run_mask(i < n ? it.tab : kNone));
         }
         wave_sync();
         Real* t = bA;
