@@ -431,7 +431,12 @@ __host__ __device__ inline size_t nodec_bytes(size_t real, int n, int nwgd) {
 // ---------------------------------------------------------------------------
 // row-invariant data of one item, cached in registers for a stage's rows
 // ---------------------------------------------------------------------------
-constexpr int kCache = 6;  // items per lane held in registers (stages of up to 384 items)
+// Items per lane held in registers by the forward (kCache) and the reverse
+// sweep (kCacheB): 128 per stage; items past them fetch their segment data
+// by lane shuffles.  Measured: larger caches (up to 384 items) were slower,
+// their unrolled per-slot code and registers cost more than they save.
+constexpr int kCache = 2;
+constexpr int kCacheB = 2;
 
 __device__ __forceinline__ ItemRow item_row(const SegInfo& sg, uint32_t i) {
   ItemRow it;
@@ -534,7 +539,7 @@ template <typename Real, bool Scaled, bool StoreAll, bool TabL>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                        const uint32_t* fam_count) {
-  constexpr int RC = 3;  // records per lane held in registers (192 per stage)
+  constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x % kWG;
@@ -1143,7 +1148,7 @@ template <typename Real, bool Scaled, bool TabL>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                         int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                         const uint32_t* fam_count) {
-  constexpr int RC = 3;  // records per lane held in registers (192 per stage)
+  constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x % kWG;
@@ -1341,14 +1346,14 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         sCoef[k] = coef_at(rb + k);
       }
       wave_sync();
-      ItemRow cache[kCache];
-      ItemTop tops[kCache];
+      ItemRow cache[kCacheB];
+      ItemTop tops[kCacheB];
       // ℓ of the rows below, loaded two rows ahead into two register sets
       // that take turns (rows r and r-1 use preA and preB)
-      Real preA[kCache], preB[kCache];
+      Real preA[kCacheB], preB[kCacheB];
       const int rtop = (int)S.r_max - 1;
 #pragma unroll
-      for (int c = 0; c < kCache; ++c) {
+      for (int c = 0; c < kCacheB; ++c) {
         preA[c] = 0;
         preB[c] = 0;
         if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
@@ -1364,9 +1369,9 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
       }
       // runs of items sharing a segment (table row), for the d/dφ, d/dψ sums
-      RunMask im[kCache];
+      RunMask im[kCacheB];
 #pragma unroll
-      for (int c = 0; c < kCache; ++c) {
+      for (int c = 0; c < kCacheB; ++c) {
         if ((uint32_t)c * kWG < n) {
           const uint32_t i = lane + c * kWG;
           im[c] = run_mask(i < n ? cache[c].tab : kNone);
@@ -1386,41 +1391,69 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         rcoef[c] = in ? sCoef[k] : Real(0);
         rm[c] = run_mask(rown[c]);
       }
-      auto row_step = [&](const int r, Real (&pre)[kCache]) {
-        // items: ℓ of row r-1, the φ part of ā(r-1) and ψ_r·ā(r)
-        auto item_a = [&](const ItemRow& it, uint32_t i, Real pv) {
-          const Real g = bA[i];
-          if (r < (int)it.R) {
-            Real ph, ps;
-            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
-            bV[i] = pv;
-            bB[i] = ph * g;
-            sT[i] = ps * g;
-          } else {
-            bB[i] = g;
-            sT[i] = 0;
-          }
-          sFo[i] = 0;
-        };
-#pragma unroll
-        for (int c = 0; c < kCache; ++c) {
-          const uint32_t i = lane + c * kWG;
-          if (i < n) item_a(cache[c], i, pre[c]);
+      // Per row r (top down) two item passes and one record pass:
+      //   A(r)    items: ℓ of row r-1 (prefetched), φ_r·ā(r) into the other
+      //           buffer (the start of ā(r-1)) and ψ_r·ā(r) for the records
+      //   REC(r)  records: the ψ part scattered into the two parts' ā(r-1),
+      //           and the owner's split sum (forcing of d/dψ_r)
+      //   C(r)    items: d/dφ_r, d/dψ_r per segment
+      // C(r) and A(r-1) share one pass over the items.
+      // A(x) for item i from ā(x) = g into dst; pv = ℓ(x-1)
+      auto item_a = [&](const int x, const ItemRow& it, uint32_t i, Real g, Real pv, Real* dst) {
+        if (x < (int)it.R) {
+          Real ph, ps;
+          slice<Real, TabL>(M, sTab, it.tab + x, ph, ps);
+          bV[i] = pv;
+          dst[i] = ph * g;
+          sT[i] = ps * g;
+        } else {
+          dst[i] = g;
+          sT[i] = 0;
         }
-        for (uint32_t base = kCache * kWG; base < n; base += kWG) {  // uniform: lane shuffles
+        sFo[i] = 0;
+      };
+      auto refill = [&](const int x, Real (&pre)[kCacheB]) {  // ℓ(x-3) for A(x-2)
+#pragma unroll
+        for (int c = 0; c < kCacheB; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n && x >= 3 && x - 2 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(x - 3) * cache[c].stride];
+        }
+      };
+      // C(r) for item i: ā(r) = g, ℓ(r-1) = v, split sum fo (all lanes take part)
+      auto item_c = [&](const int r, const ItemRow& it, bool on, Real g, Real v, Real fo, const RunMask& rmk) {
+        Real cphi = on ? g * v : Real(0);
+        Real cpsi = on ? g * fo : Real(0);
+        cphi = run_sum(cphi, rmk);
+        cpsi = run_sum(cpsi, rmk);
+        if (rmk.last && on) {
+          const uint32_t key = it.tab;
+          if (TabL) {
+            if (cphi != Real(0)) lds_add(&accPS[AL.phi + key + r], cphi);
+            if (cpsi != Real(0)) lds_add(&accPS[AL.psi + key + r], cpsi);
+          } else {
+            acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
+            acc_add<false>(W.acc + AL.psi + key + r, (double)cpsi);
+          }
+        }
+      };
+      // A(rtop) on its own
+      {
+#pragma unroll
+        for (int c = 0; c < kCacheB; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n) item_a(rtop, cache[c], i, bA[i], preA[c], bB);
+        }
+        for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {  // uniform: lane shuffles
           const uint32_t i = base + lane;
           const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
-          if (i < n) item_a(it, i, r < (int)it.R ? ell[it.cell + (size_t)(r - 1) * it.stride] : Real(0));
+          if (i < n)
+            item_a(rtop, it, i, bA[i], rtop < (int)it.R ? ell[it.cell + (size_t)(rtop - 1) * it.stride] : Real(0), bB);
         }
-#pragma unroll
-        for (int c = 0; c < kCache; ++c) {
-          const uint32_t i = lane + c * kWG;
-          if (i < n && r >= 3 && r - 2 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(r - 3) * cache[c].stride];
-        }
-        wave_sync();
-        // records, one per lane: the ψ part scattered into the two parts, and
-        // the split sum of the owner (forcing of d/dψ_r).  The cached records'
-        // reads are all issued before any is used.
+        refill(rtop, preA);
+      }
+      auto rec_pass = [&]() {
+        // records, one per lane; the cached records' reads are all issued
+        // before any is used
         {
           Real va[RC], vb[RC], g[RC];
 #pragma unroll
@@ -1465,52 +1498,42 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           fo = row_seg_sum(fo, own, &last);
           if (last && own != kNone && fo != Real(0)) lds_add(&sFo[own], fo);
         }
+      };
+      // REC(r), then C(r) with A(r-1); A(r-1) reads ℓ(r-2) from `pre`
+      auto row_step = [&](const int r, Real (&pre)[kCacheB]) {
         wave_sync();
-        // d/dφ_r, d/dψ_r per segment (all lanes take part: segmented reductions)
-        // (the items of one segment share its table row: sums over each
-        // 16-lane run of them, one LDS atomic per run)
-        auto item_c = [&](const ItemRow& it, uint32_t i, bool valid, const RunMask& rmk) {
-          Real cphi = 0, cpsi = 0;
-          const bool on = valid && r < (int)it.R;
-          if (on) {
-            const Real g = bA[i];
-            cphi = g * bV[i];
-            cpsi = g * sFo[i];
-          }
-          cphi = run_sum(cphi, rmk);
-          cpsi = run_sum(cpsi, rmk);
-          if (rmk.last && on) {
-            const uint32_t key = it.tab;
-            if (TabL) {
-              if (cphi != Real(0)) lds_add(&accPS[AL.phi + key + r], cphi);
-              if (cpsi != Real(0)) lds_add(&accPS[AL.psi + key + r], cpsi);
-            } else {
-              acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
-              acc_add<false>(W.acc + AL.psi + key + r, (double)cpsi);
-            }
-          }
-        };
+        rec_pass();
+        wave_sync();
+        const bool more = r >= 2;  // uniform
 #pragma unroll
-        for (int c = 0; c < kCache; ++c) {
+        for (int c = 0; c < kCacheB; ++c) {
           if ((uint32_t)c * kWG < n) {  // uniform: the run sums use DPP
             const uint32_t i = lane + c * kWG;
-            item_c(cache[c], i, i < n, im[c]);
+            const uint32_t ii = i < n ? i : 0u;
+            const Real g = bA[ii], v = bV[ii], fo = sFo[ii], g1 = bB[ii];
+            item_c(r, cache[c], i < n && r < (int)cache[c].R, g, v, fo, im[c]);
+            if (more && i < n) item_a(r - 1, cache[c], i, g1, pre[c], bA);
           }
         }
-        for (uint32_t base = kCache * kWG; base < n; base += kWG) {
+        for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {  // uniform: lane shuffles, DPP
           const uint32_t i = base + lane;
+          const uint32_t ii = i < n ? i : 0u;
           const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
-          item_c(it, i, i < n, run_mask(i < n ? it.tab : kNone));
+          const Real g = bA[ii], v = bV[ii], fo = sFo[ii], g1 = bB[ii];
+          item_c(r, it, i < n && r < (int)it.R, g, v, fo, run_mask(i < n ? it.tab : kNone));
+          if (more && i < n)
+            item_a(r - 1, it, i, g1, r - 1 < (int)it.R ? ell[it.cell + (size_t)(r - 2) * it.stride] : Real(0), bA);
         }
-        wave_sync();
+        if (more) refill(r - 1, pre);
         Real* t = bA;
         bA = bB;
         bB = t;
       };
       for (int r = rtop; r >= 1; r -= 2) {
-        row_step(r, preA);
-        if (r >= 2) row_step(r - 1, preB);
+        row_step(r, preB);
+        if (r >= 2) row_step(r - 1, preA);
       }
+      wave_sync();
       if (TabL) {
         // move this stage's d/dφ_r, d/dψ_r into the global FP64 sums (each LDS
         // partial then holds at most one stage's contributions per family)
@@ -1535,14 +1558,14 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
       }
       // row 0: adjoints of the child tops, ε and q
-      // pass over all items: cached tops for the first kCache per lane
+      // pass over all items: cached tops for the first kCacheB per lane
       auto each = [&](auto&& fn) {
 #pragma unroll
-        for (int c = 0; c < kCache; ++c) {
+        for (int c = 0; c < kCacheB; ++c) {
           const uint32_t i = lane + c * kWG;
           if (i < n) fn(tops[c], i);
         }
-        for (uint32_t base = kCache * kWG; base < n; base += kWG) {
+        for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {
           const uint32_t i = base + lane;
           const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
           if (i < n) fn(item_top(sg, i), i);
