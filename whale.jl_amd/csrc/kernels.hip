@@ -245,41 +245,38 @@ __device__ __forceinline__ Real row_seg_sum(Real v, uint32_t key, bool* last) {
 // found once per stage.  Each row's sum is then at most four DPP adds under
 // those lane masks, and none past what the longest run needs (most items
 // have one or two splits).
+// The masks are held as 0/1 multipliers (one FMA per shift, no lane masks
+// to keep in scalar registers); a term past the range (inf) can then spread
+// a NaN within its 16 lanes, which the range checks catch like the inf.
 struct RunMask {
-  bool s1, s2, s4, s8, last;
-  int steps;  // wave-uniform
+  float s1, s2, s4, s8;  // 1 where the shift stays inside the lane's run
+  uint32_t last;         // 1 where the lane ends its run
+  int steps;             // shifts the longest run needs (wave-uniform)
+  __device__ bool ends() const { return last != 0u; }
 };
 __device__ __forceinline__ RunMask run_mask(uint32_t key) {
-  RunMask m;
-  m.s1 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x111, 0xf, 0xf, false) == key;
-  m.s2 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x112, 0xf, 0xf, false) == key;
-  m.s4 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x114, 0xf, 0xf, false) == key;
-  m.s8 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x118, 0xf, 0xf, false) == key;
+  const bool b1 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x111, 0xf, 0xf, false) == key;
+  const bool b2 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x112, 0xf, 0xf, false) == key;
+  const bool b4 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x114, 0xf, 0xf, false) == key;
+  const bool b8 = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x118, 0xf, 0xf, false) == key;
   const uint32_t kn = (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)key, 0x101, 0xf, 0xf, false);  // row_shl:1
-  m.last = key != kNone && (kn != key || (threadIdx.x & 15) == 15);
-  // a run reaching back 2^k lanes also reaches back every shorter distance
   const bool real = key != kNone;
-  m.steps = __any(real && m.s8) ? 4 : __any(real && m.s4) ? 3 : __any(real && m.s2) ? 2 : __any(real && m.s1) ? 1 : 0;
+  RunMask m;
+  m.s1 = b1 ? 1.0f : 0.0f;
+  m.s2 = b2 ? 1.0f : 0.0f;
+  m.s4 = b4 ? 1.0f : 0.0f;
+  m.s8 = b8 ? 1.0f : 0.0f;
+  m.last = (real && (kn != key || (threadIdx.x & 15) == 15)) ? 1u : 0u;
+  // a run reaching back 2^k lanes also reaches back every shorter distance
+  m.steps = __any(real && b8) ? 4 : __any(real && b4) ? 3 : __any(real && b2) ? 2 : __any(real && b1) ? 1 : 0;
   return m;
 }
 template <typename Real>
 __device__ __forceinline__ Real run_sum(Real v, const RunMask& m) {
-  if (m.steps > 0) {
-    const Real u = dpp_shr<1>(v);
-    if (m.s1) v += u;
-  }
-  if (m.steps > 1) {
-    const Real u = dpp_shr<2>(v);
-    if (m.s2) v += u;
-  }
-  if (m.steps > 2) {
-    const Real u = dpp_shr<4>(v);
-    if (m.s4) v += u;
-  }
-  if (m.steps > 3) {
-    const Real u = dpp_shr<8>(v);
-    if (m.s8) v += u;
-  }
+  if (m.steps > 0) v = fma(dpp_shr<1>(v), (Real)m.s1, v);
+  if (m.steps > 1) v = fma(dpp_shr<2>(v), (Real)m.s2, v);
+  if (m.steps > 2) v = fma(dpp_shr<4>(v), (Real)m.s4, v);
+  if (m.steps > 3) v = fma(dpp_shr<8>(v), (Real)m.s8, v);
   return v;
 }
 
@@ -795,7 +792,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
           for (int c = 0; c < RC; ++c) {
             if ((uint32_t)c * kWG < nrec) {  // uniform: the row sums use DPP
               const Real t = run_sum(rcoef[c] * va[c] * vb[c], rm[c]);
-              if (rm[c].last && t != Real(0)) lds_add(&sAcc[rown[c]], t);
+              if (rm[c].ends() && t != Real(0)) lds_add(&sAcc[rown[c]], t);
             }
           }
         }
@@ -1425,7 +1422,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         Real cpsi = on ? g * fo : Real(0);
         cphi = run_sum(cphi, rmk);
         cpsi = run_sum(cpsi, rmk);
-        if (rmk.last && on) {
+        if (rmk.ends() && on) {
           const uint32_t key = it.tab;
           if (TabL) {
             if (cphi != Real(0)) lds_add(&accPS[AL.phi + key + r], cphi);
@@ -1473,7 +1470,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
                 lds_add(&bB[rops[c] >> 16], t * va[c]);
               }
               const Real fo = run_sum(rcoef[c] * va[c] * vb[c], rm[c]);
-              if (rm[c].last && fo != Real(0)) lds_add(&sFo[rown[c]], fo);
+              if (rm[c].ends() && fo != Real(0)) lds_add(&sFo[rown[c]], fo);
             }
           }
         }
