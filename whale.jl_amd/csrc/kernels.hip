@@ -1396,17 +1396,15 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
       //   C(r)    items: d/dφ_r, d/dψ_r per segment
       // C(r) and A(r-1) share one pass over the items.
       // A(x) for item i from ā(x) = g into dst; pv = ℓ(x-1)
+      // (branch-free: past its segment's rows an item passes ā through and
+      // its records' forcing is zero; pv is then a finite stale value)
       auto item_a = [&](const int x, const ItemRow& it, uint32_t i, Real g, Real pv, Real* dst) {
-        if (x < (int)it.R) {
-          Real ph, ps;
-          slice<Real, TabL>(M, sTab, it.tab + x, ph, ps);
-          bV[i] = pv;
-          dst[i] = ph * g;
-          sT[i] = ps * g;
-        } else {
-          dst[i] = g;
-          sT[i] = 0;
-        }
+        const bool in = x < (int)it.R;
+        Real ph, ps;
+        slice<Real, TabL>(M, sTab, in ? it.tab + x : 0u, ph, ps);
+        bV[i] = pv;
+        dst[i] = in ? ph * g : g;
+        sT[i] = in ? ps * g : Real(0);
         sFo[i] = 0;
       };
       auto refill = [&](const int x, Real (&pre)[kCacheB]) {  // ℓ(x-3) for A(x-2)
@@ -1422,11 +1420,11 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         Real cpsi = on ? g * fo : Real(0);
         cphi = run_sum(cphi, rmk);
         cpsi = run_sum(cpsi, rmk);
-        if (rmk.ends() && on) {
+        if (rmk.ends() && on && (cphi != Real(0) || cpsi != Real(0))) {
           const uint32_t key = it.tab;
           if (TabL) {
-            if (cphi != Real(0)) lds_add(&accPS[AL.phi + key + r], cphi);
-            if (cpsi != Real(0)) lds_add(&accPS[AL.psi + key + r], cpsi);
+            lds_add(&accPS[AL.phi + key + r], cphi);
+            lds_add(&accPS[AL.psi + key + r], cpsi);
           } else {
             acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
             acc_add<false>(W.acc + AL.psi + key + r, (double)cpsi);
