@@ -976,63 +976,93 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
         }
       }
       const LevelBounds LB(B, F, lane);
+      // bA accumulates each clade's root value: it starts at the direct term
+      // cB·(F(γ)ε_g + G(γ)ε_f) (zero for a clade without a value below the
+      // root), then the level passes add the split terms of its records.
+      for (uint32_t i = lane; i < n; i += kWG) bA[i] = cB * (bF[i] * eg + bB[i] * ef);
       wave_sync();
+      // record k of a level: operands, p, owner
+      auto rrec = [&](uint32_t k, uint32_t& ops, Real& p, uint32_t& own) {
+        if (inl) { ops = sOps[k]; p = sCoef[k]; own = sOwn[k]; }
+        else { ops = B.rec_ops[rb + k]; p = (Real)B.rec_p[rb + k]; own = B.rec_own[rb + k]; }
+      };
+      // Scaled forms: a clade still without an exponent at its level ("open")
+      // takes the largest exponent of its split terms.  The first record pass
+      // raises sEi[own] to kOpen + that exponent (above every real exponent,
+      // so the open owners stay recognisable); the second adds each term
+      // rescaled to its owner's exponent; the item pass normalises the open
+      // clades' sums and fixes their exponents.
+      constexpr int kOpen = 1 << 28;
       for (uint32_t Lv = 0; Lv < F.n_lvl; ++Lv) {
         const uint32_t a0 = LB.at(B, F, Lv), a1 = LB.at(B, F, Lv + 1);
-        for (uint32_t i = a0 + lane; i < a1; i += kWG) {
-          const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
-          Real v;
-          if (!Scaled) {
-            Real a = 0, b = 0;
-            for (uint32_t k = r0; k < r1; ++k) {
-              const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
-              const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
-              const uint32_t x = ops & 0xffff, y = ops >> 16;
-              a += p * bA[x] * bA[y];
-              b += p * (bF[x] * bB[y] + bB[x] * bF[y]);
-            }
-            const Real c = bF[i] * eg + bB[i] * ef;
-            v = cA * a + cB * (b + c);
-          } else {
-            const int Eg = sEi[i];
-            if (Eg != kUndef) {
-              // exponent fixed: split terms rescaled to it, summed in FP64
-              double sacc = 0.0;
-              for (uint32_t k = r0; k < r1; ++k) {
-                const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
-                const Real pk = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
-                const uint32_t x = ops & 0xffff, y = ops >> 16;
-                const int ex_ = sEi[x], ey = sEi[y];
-                if (ex_ == kUndef || ey == kUndef) continue;
-                const double t = (double)cA * bA[x] * bA[y] + (double)cB * ((double)bF[x] * bB[y] + (double)bB[x] * bF[y]);
-                sacc += ldexp((double)pk * t, ex_ + ey - Eg);
-              }
-              const Real c = bF[i] * eg + bB[i] * ef;
-              v = cB * c + (Real)sacc;
+        const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sRO[a0]);
+        const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sRO[a1]);
+        if (k0 == k1) continue;  // clades without splits keep their direct term
+        if (Scaled) {
+          for (uint32_t k = k0 + lane; k < k1; k += kWG) {
+            uint32_t ops, own;
+            Real pk;
+            rrec(k, ops, pk, own);
+            const int Eo = sEi[own];
+            if (Eo != kUndef && Eo < kOpen / 2) continue;  // exponent fixed below the root
+            const uint32_t x = ops & 0xffff, y = ops >> 16;
+            const int ex_ = sEi[x], ey = sEi[y];
+            int emax = kUndef;
+            Real m;
+            int ee;
+            if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+            if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+            if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+            if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
+          }
+          wave_sync();
+        }
+        for (uint32_t base = k0; base < k1; base += kWG) {  // uniform: the run sums use DPP
+          const uint32_t k = base + lane;
+          uint32_t own = kNone;
+          Real t = 0;
+          if (k < k1) {
+            uint32_t ops;
+            Real pk;
+            rrec(k, ops, pk, own);
+            const uint32_t x = ops & 0xffff, y = ops >> 16;
+            if (!Scaled) {
+              t = pk * (cA * bA[x] * bA[y] + cB * (bF[x] * bB[y] + bB[x] * bF[y]));
             } else {
-              XSum<Real> xs;
-              for (uint32_t k = r0; k < r1; ++k) {
-                const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
-                const Real pk = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
-                const uint32_t x = ops & 0xffff, y = ops >> 16;
-                const int ex_ = sEi[x], ey = sEi[y];
+              const int Eo = sEi[own], ex_ = sEi[x], ey = sEi[y];
+              if (Eo >= kOpen / 2) {
+                // open owner: each product as (mantissa, exponent) against its largest
+                const int K = Eo - kOpen;
                 Real m;
                 int ee;
-                if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); xs.add(m, ee); }
-                if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); xs.add(m, ee); }
-                if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); xs.add(m, ee); }
-              }
-              v = 0;
-              if (xs.k != kUndef) {
-                int ex;
-                v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
-                sEi[i] = xs.k + ex + kBirth;  // no clade of this level is a part of another
+                if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
+                if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
+                if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
+              } else if (Eo != kUndef && ex_ != kUndef && ey != kUndef) {
+                // fixed exponent: the term rescaled to it, formed in FP64 (wide
+                // enough for any product of the stored values)
+                const double tt = (double)cA * bA[x] * bA[y] + (double)cB * ((double)bF[x] * bB[y] + (double)bB[x] * bF[y]);
+                t = (Real)ldexp((double)pk * tt, ex_ + ey - Eo);
               }
             }
           }
-          bA[i] = v;
+          bool last;
+          t = row_seg_sum(t, own, &last);
+          if (last && own != kNone && t != Real(0)) lds_add(&bA[own], t);
         }
         wave_sync();
+        if (Scaled) {
+          for (uint32_t i = a0 + lane; i < a1; i += kWG) {
+            const int Eo = sEi[i];
+            if (Eo >= kOpen / 2) {
+              int e2;
+              const Real mm = frexp_r(bA[i], &e2);
+              bA[i] = ldexp_r(mm, -kBirth);
+              sEi[i] = Eo - kOpen + e2 + kBirth;
+            }
+          }
+          wave_sync();
+        }
       }
       for (uint32_t i = lane; i < n; i += kWG) {
         ell[sg.ell + i] = bA[i];
