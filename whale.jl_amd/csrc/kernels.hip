@@ -617,11 +617,90 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
         sP[k] = p;
       }
       wave_sync();
+      // Row 0 of a stage without WGD segments: the split terms are summed
+      // record-parallel first (into sAcc, which is zero until the row loop),
+      // so the item pass below only adds the direct terms.  A clade still
+      // without an exponent ("open") takes the largest exponent of its terms:
+      // the first record pass raises sEi[own] to kOpen + that exponent, above
+      // every real exponent; the item pass normalises and fixes it.
+      constexpr int kOpen = 1 << 28;
+      const bool fast0 = ST.inreg && !__any(lane < S.n_segs && ST.mine.kind == WHALE_NODE_WGD);
+      if (fast0) {
+        if (Scaled) {
+          for (uint32_t k = lane; k < nrec; k += kWG) {
+            const uint32_t own = sOwn[k];
+            const int Eo = sEi[own];
+            if (Eo != kUndef && Eo < kOpen / 2) continue;
+            const uint32_t ops = sOps[k];
+            const uint32_t a = ops & 0xffff, b = ops >> 16;
+            const Real pk = sCoef[k];
+            const Real fa = bF[a], fb = bF[b], ga = bB[a], gb = bB[b];
+            const int ea = sEi[a], eb = sEi[b];
+            int emax = kUndef;
+            Real m;
+            int ee, em;
+            if (fa != Real(0) && gb != Real(0)) { xprod(pk, fa, ea, gb, eb, &m, &ee); (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+            if (ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ee); (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+            if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
+          }
+          wave_sync();
+        }
+        for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the run sums use DPP
+          const uint32_t k = base + lane;
+          uint32_t own = kNone;
+          Real t = 0;
+          if (k < nrec) {
+            own = sOwn[k];
+            const uint32_t ops = sOps[k];
+            const uint32_t a = ops & 0xffff, b = ops >> 16;
+            const Real pk = sCoef[k];
+            const Real fa = bF[a], fb = bF[b], ga = bB[a], gb = bB[b];
+            if (!Scaled) {
+              t = pk * (fa * gb + ga * fb);
+            } else {
+              const int Eo = sEi[own], ea = sEi[a], eb = sEi[b];
+              if (Eo >= kOpen / 2) {
+                const int K = Eo - kOpen;
+                Real m;
+                int ee;
+                if (fa != Real(0) && gb != Real(0)) { xprod(pk, fa, ea, gb, eb, &m, &ee); t += ldexp_r(m, ee - K); }
+                if (ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ee); t += ldexp_r(m, ee - K); }
+              } else if (Eo != kUndef && ea != kUndef && eb != kUndef) {
+                // fixed exponent: the term formed in FP64 (wide enough for any
+                // product of the stored values), rescaled to it
+                t = (Real)ldexp((double)pk * ((double)fa * gb + (double)ga * fb), ea + eb - Eo);
+              }
+            }
+          }
+          bool last;
+          t = row_seg_sum(t, own, &last);
+          if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
+        }
+        wave_sync();
+      }
       // row 0: terms from the child branches (whale!/whalewgd!, src/core.jl:95-128)
       auto row0 = [&](const SegInfo& sg, uint32_t i) {
         const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
         Real v = 0;
-        if (sg.kind == WHALE_NODE_LEAF) {
+        if (fast0 && sg.kind != WHALE_NODE_LEAF) {
+          // speciation: the split terms are in sAcc
+          const Real sp = sAcc[i];
+          sAcc[i] = 0;
+          const Real ef = NC.etop[sg.c0], eg = NC.etop[sg.c1];
+          if (!Scaled) {
+            v = sp + (bF[i] * eg + bB[i] * ef);
+          } else {
+            const int Eg = sEi[i];
+            if (Eg >= kOpen / 2) {
+              int e2;
+              v = ldexp_r(frexp_r(sp, &e2), -kBirth);
+              sEi[i] = Eg - kOpen + e2 + kBirth;
+              sFlag[i] = kYoung;
+            } else if (Eg != kUndef) {
+              v = (bF[i] * eg + bB[i] * ef) + sp;
+            }
+          }
+        } else if (sg.kind == WHALE_NODE_LEAF) {
           if (r1 == r0) {  // a clade of one gene (every larger clade has a split)
             const Real lp = NC.leafP[sg.node];
             if (Scaled) {
