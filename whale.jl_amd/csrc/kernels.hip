@@ -400,6 +400,11 @@ struct SegTab {
   // the row-loop data of item i of segment `sl` (items past the register
   // cache); every lane of the wave must call this together
   __device__ __forceinline__ ItemRow row(const BatchDev& B, const ModelDev& M, uint32_t sl, uint32_t i) const;
+  // first table row of segment `sl`'s node; every lane must call this together
+  __device__ __forceinline__ uint32_t tab(const BatchDev& B, const ModelDev& M, uint32_t sl) const {
+    if (!inreg) return (uint32_t)M.node[B.seg[seg0 + sl].node].tab;
+    return __shfl(mine.tab, (int)sl, kWG);
+  }
 };
 
 // per-node constants of the current parameters in LDS: ε at the top of each
@@ -1429,7 +1434,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
       Real* bB = T.bB;  // next row (and ī sums at row 0)
       Real* const bV = T.bV;  // ℓ of the row below / child0 tops
       Real* const bG = T.bG;  // child1 tops; ψ·ā per item in the row loop
-      Real* const sFo = T.sFo;  // split sums Σ c·ℓ(γ1)ℓ(γ2) per item (row loop)
+      Real* const sFo = T.sFo;  // per-item sums and factors of the row-0 step
       Real* const sCoef = T.sCoef;
       uint32_t* const sOps = T.sOps;
       uint16_t* const sOwn = T.sOwn;
@@ -1484,10 +1489,12 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
       }
       Real* const sT = bG;  // ψ_r·ā(r) per item, for the record pass
-      // the first RC records per lane in registers: operands, owner, coefficient
-      uint32_t rops[RC], rown[RC];
+      // the first RC records per lane in registers: operands, owner,
+      // coefficient, and the table row of the owner's segment with the runs
+      // of records sharing it (the records of a segment are consecutive)
+      uint32_t rops[RC], rown[RC], rtab[RC];
       Real rcoef[RC];
-      RunMask rm[RC];
+      RunMask rs[RC];
 #pragma unroll
       for (int c = 0; c < RC; ++c) {
         const uint32_t k = lane + c * kWG;
@@ -1495,8 +1502,19 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         rops[c] = in ? sOps[k] : 0u;
         rown[c] = in ? (uint32_t)sOwn[k] : kNone;
         rcoef[c] = in ? sCoef[k] : Real(0);
-        rm[c] = run_mask(rown[c]);
+        if ((uint32_t)c * kWG < nrec) {  // uniform: lane shuffles
+          const uint32_t t = ST.tab(B, M, in ? (uint32_t)sSegL[rown[c]] : 0u);
+          rtab[c] = in ? t : kNone;
+          rs[c] = run_mask(rtab[c]);
+        }
       }
+      // d/dψ_r: Σ over a segment's records of ā(r)[owner]·c·ℓ(γ1)ℓ(γ2), one
+      // atomic per run of records (records past their segment's rows have
+      // ψ_r·ā = 0 and add nothing)
+      auto add_psi = [&](uint32_t key, int r, Real x) {
+        if (TabL) lds_add(&accPS[AL.psi + key + r], x);
+        else acc_add<false>(W.acc + AL.psi + key + r, (double)x);
+      };
       // Per row r (top down) two item passes and one record pass:
       //   A(r)    items: ℓ of row r-1 (prefetched), φ_r·ā(r) into the other
       //           buffer (the start of ā(r-1)) and ψ_r·ā(r) for the records
@@ -1514,7 +1532,6 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         bV[i] = pv;
         dst[i] = in ? ph * g : g;
         sT[i] = in ? ps * g : Real(0);
-        sFo[i] = 0;
       };
       auto refill = [&](const int x, Real (&pre)[kCacheB]) {  // ℓ(x-3) for A(x-2)
 #pragma unroll
@@ -1523,21 +1540,14 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           if (i < n && x >= 3 && x - 2 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(x - 3) * cache[c].stride];
         }
       };
-      // C(r) for item i: ā(r) = g, ℓ(r-1) = v, split sum fo (all lanes take part)
-      auto item_c = [&](const int r, const ItemRow& it, bool on, Real g, Real v, Real fo, const RunMask& rmk) {
+      // C(r) for item i: d/dφ_r from ā(r) = g and ℓ(r-1) = v (all lanes take part)
+      auto item_c = [&](const int r, const ItemRow& it, bool on, Real g, Real v, const RunMask& rmk) {
         Real cphi = on ? g * v : Real(0);
-        Real cpsi = on ? g * fo : Real(0);
         cphi = run_sum(cphi, rmk);
-        cpsi = run_sum(cpsi, rmk);
-        if (rmk.ends() && on && (cphi != Real(0) || cpsi != Real(0))) {
+        if (rmk.ends() && on && cphi != Real(0)) {
           const uint32_t key = it.tab;
-          if (TabL) {
-            lds_add(&accPS[AL.phi + key + r], cphi);
-            lds_add(&accPS[AL.psi + key + r], cpsi);
-          } else {
-            acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
-            acc_add<false>(W.acc + AL.psi + key + r, (double)cpsi);
-          }
+          if (TabL) lds_add(&accPS[AL.phi + key + r], cphi);
+          else acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
         }
       };
       // A(rtop) on its own
@@ -1555,58 +1565,62 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
         refill(rtop, preA);
       }
-      auto rec_pass = [&]() {
+      auto rec_pass = [&](const int r) {
         // records, one per lane; the cached records' reads are all issued
         // before any is used
         {
-          Real va[RC], vb[RC], g[RC];
+          Real va[RC], vb[RC], g[RC], ga[RC];
 #pragma unroll
           for (int c = 0; c < RC; ++c) {
             if ((uint32_t)c * kWG < nrec) {
+              const uint32_t o = rown[c] != kNone ? rown[c] : 0u;
               va[c] = bV[rops[c] & 0xffff];
               vb[c] = bV[rops[c] >> 16];
-              g[c] = rown[c] != kNone ? sT[rown[c]] : Real(0);
+              g[c] = rown[c] != kNone ? sT[o] : Real(0);
+              ga[c] = bA[o];
             }
           }
 #pragma unroll
           for (int c = 0; c < RC; ++c) {
-            if ((uint32_t)c * kWG < nrec) {  // uniform: the row sums use DPP
+            if ((uint32_t)c * kWG < nrec) {  // uniform: the run sums use DPP
               const Real t = g[c] * rcoef[c];
               if (t != Real(0)) {
                 lds_add(&bB[rops[c] & 0xffff], t * vb[c]);
                 lds_add(&bB[rops[c] >> 16], t * va[c]);
               }
-              const Real fo = run_sum(rcoef[c] * va[c] * vb[c], rm[c]);
-              if (rm[c].ends() && fo != Real(0)) lds_add(&sFo[rown[c]], fo);
+              const Real x = run_sum(t != Real(0) ? ga[c] * rcoef[c] * va[c] * vb[c] : Real(0), rs[c]);
+              if (rs[c].ends() && x != Real(0)) add_psi(rtab[c], r, x);
             }
           }
         }
-        for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: the row sums use DPP
+        for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: shuffles, row sums
           const uint32_t k = base + lane;
-          uint32_t own = kNone;
-          Real fo = 0;
-          if (k < nrec) {
+          const bool in = k < nrec;
+          const uint32_t own = in ? (uint32_t)sOwn[k] : 0u;
+          const uint32_t tb = ST.tab(B, M, sSegL[own]);
+          const uint32_t key = in ? tb : kNone;
+          Real x = 0;
+          if (in) {
             const uint32_t ops = sOps[k];
             const uint32_t a = ops & 0xffff, b = ops >> 16;
-            own = sOwn[k];
             const Real c = sCoef[k];
             const Real va = bV[a], vb = bV[b];
-            fo = c * va * vb;
             const Real t = sT[own] * c;
             if (t != Real(0)) {
               lds_add(&bB[a], t * vb);
               lds_add(&bB[b], t * va);
+              x = bA[own] * c * va * vb;
             }
           }
           bool last;
-          fo = row_seg_sum(fo, own, &last);
-          if (last && own != kNone && fo != Real(0)) lds_add(&sFo[own], fo);
+          x = row_seg_sum(x, key, &last);
+          if (last && key != kNone && x != Real(0)) add_psi(key, r, x);
         }
       };
       // REC(r), then C(r) with A(r-1); A(r-1) reads ℓ(r-2) from `pre`
       auto row_step = [&](const int r, Real (&pre)[kCacheB]) {
         wave_sync();
-        rec_pass();
+        rec_pass(r);
         wave_sync();
         const bool more = r >= 2;  // uniform
 #pragma unroll
@@ -1614,8 +1628,8 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           if ((uint32_t)c * kWG < n) {  // uniform: the run sums use DPP
             const uint32_t i = lane + c * kWG;
             const uint32_t ii = i < n ? i : 0u;
-            const Real g = bA[ii], v = bV[ii], fo = sFo[ii], g1 = bB[ii];
-            item_c(r, cache[c], i < n && r < (int)cache[c].R, g, v, fo, im[c]);
+            const Real g = bA[ii], v = bV[ii], g1 = bB[ii];
+            item_c(r, cache[c], i < n && r < (int)cache[c].R, g, v, im[c]);
             if (more && i < n) item_a(r - 1, cache[c], i, g1, pre[c], bA);
           }
         }
@@ -1623,8 +1637,8 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           const uint32_t i = base + lane;
           const uint32_t ii = i < n ? i : 0u;
           const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
-          const Real g = bA[ii], v = bV[ii], fo = sFo[ii], g1 = bB[ii];
-          item_c(r, it, i < n && r < (int)it.R, g, v, fo, run_mask(i < n ? it.tab : kNone));
+          const Real g = bA[ii], v = bV[ii], g1 = bB[ii];
+          item_c(r, it, i < n && r < (int)it.R, g, v, run_mask(i < n ? it.tab : kNone));
           if (more && i < n)
             item_a(r - 1, it, i, g1, r - 1 < (int)it.R ? ell[it.cell + (size_t)(r - 2) * it.stride] : Real(0), bA);
         }
