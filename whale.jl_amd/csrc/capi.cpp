@@ -785,9 +785,9 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     constexpr int kLanes = 3;
     hipStream_t lanes[kLanes] = {st, c->side, c->side3};
     // classes in launch order and their streams
-    static const int kOrder[whale_batch::kClasses] = {5, 4, 1, 3, 2, 0};
-    static const int kLane[whale_batch::kClasses] = {2, 2, 1, 0, 1, 0};  // by class
-    auto lane_of = [](int k) { return kLane[k]; };
+    static const int kOrder[2][whale_batch::kClasses] = {{5, 4, 1, 3, 2, 0}, {5, 4, 1, 3, 2, 0}};
+    static const int kLane[2][whale_batch::kClasses] = {{2, 2, 1, 0, 1, 0}, {2, 2, 1, 0, 1, 0}};  // by class
+    auto lane_of = [&](int k) { return kLane[dir][k]; };
     bool used[kLanes] = {};
     for (int k = 0; k < whale_batch::kClasses; ++k)
       if (b->cls_hi[k] > b->cls_lo[k]) used[lane_of(k)] = true;
@@ -795,7 +795,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     for (int l = 1; l < kLanes; ++l)
       if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], c->fork[3 * dir], 0));
     for (int j = 0; j < whale_batch::kClasses; ++j) {
-      const int k = kOrder[j];
+      const int k = kOrder[dir][j];
       if (b->cls_hi[k] == b->cls_lo[k]) continue;
       hipStream_t sk = lanes[lane_of(k)];
       WorkRaw Wk = W;
