@@ -722,7 +722,7 @@ int plan_batch_launches(whale_batch* b) {
 // routed to FP64 last.  Then the launch shapes.
 int assign_classes(whale_batch* b) {
   const PlanHost& P = b->plan;
-  const size_t bounds[whale_batch::kClasses - 1] = {8 * 1024, 12 * 1024, 16 * 1024, 24 * 1024, 32 * 1024};
+  const size_t bounds[whale_batch::kClasses - 1] = {10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024, 48 * 1024};
   std::vector<uint32_t> cls[whale_batch::kClasses], rt;
   for (uint32_t f : P.cost_order) {
     if (b->routed[f]) {
