@@ -811,6 +811,45 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
       }
     return WHALE_OK;
   };
+  // logL and gradient: each stream runs its classes' forwards, then their
+  // reverse sweeps, with no join between (a family's reverse sweep needs only
+  // its own forward), so the reverse sweeps of one stream overlap the last
+  // forwards of the others instead of waiting for the slowest forward of the
+  // batch.
+  auto interleaved = [&]() -> int {
+    constexpr int kLanes = 3;
+    hipStream_t lanes[kLanes] = {st, c->side, c->side3};
+    static const int kOrder[whale_batch::kClasses] = {5, 4, 1, 3, 2, 0};
+    static const int kLane[whale_batch::kClasses] = {2, 2, 1, 0, 1, 0};  // by class
+    bool used[kLanes] = {};
+    for (int k = 0; k < whale_batch::kClasses; ++k)
+      if (b->cls_hi[k] > b->cls_lo[k]) used[kLane[k]] = true;
+    HIPCHK(hipEventRecord(c->fork[0], st));
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], c->fork[0], 0));
+    for (int j = 0; j < whale_batch::kClasses; ++j) {
+      const int k = kOrder[j];
+      if (b->cls_hi[k] == b->cls_lo[k]) continue;
+      hipStream_t sk = lanes[kLane[k]];
+      WorkRaw Wk = W;
+      Wk.ticket = W.ticket + k;
+      HIPCHK(launch_forward(mode, store_all, b->cfg[mode][0][k], sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
+    }
+    for (int j = 0; j < whale_batch::kClasses; ++j) {
+      const int k = kOrder[j];
+      if (b->cls_hi[k] == b->cls_lo[k]) continue;
+      hipStream_t sk = lanes[kLane[k]];
+      WorkRaw Wk = W;
+      Wk.ticket = W.ticket + whale_batch::kClasses + k;
+      HIPCHK(launch_backward(mode, b->cfg[mode][1][k], sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
+    }
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) {
+        HIPCHK(hipEventRecord(c->fork[3 + l], lanes[l]));
+        HIPCHK(hipStreamWaitEvent(st, c->fork[3 + l], 0));
+      }
+    return WHALE_OK;
+  };
   auto single = [&](int dir, uint32_t f, const WorkRaw& Wf) -> int {
     LaunchCfg C;
     if (!plan_families(b, dir == 0, mode, &f, 1, 1, false, &C))
@@ -836,7 +875,11 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     }
     HIPCHK(hipEventRecord(c->fork[7], c->side2));
   }
-  if (launch == WHALE_LAUNCH_BATCHED) {
+  // per-kernel timing keeps the forward and reverse phases apart
+  const bool inter = launch == WHALE_LAUNCH_BATCHED && grad && !timed;
+  if (inter) {
+    if ((rc = interleaved()) != WHALE_OK) return rc;
+  } else if (launch == WHALE_LAUNCH_BATCHED) {
     if ((rc = batched(0)) != WHALE_OK) return rc;
   } else {
     for (uint32_t k = 0; k < nf; ++k)
@@ -848,7 +891,9 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   }
   if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
   if (grad) {
-    if (launch == WHALE_LAUNCH_BATCHED) {
+    if (inter) {
+      // the reverse sweeps of the classes ran above
+    } else if (launch == WHALE_LAUNCH_BATCHED) {
       if ((rc = batched(1)) != WHALE_OK) return rc;
     } else {
       for (uint32_t k = 0; k < nf; ++k)
