@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <numeric>
 #include <thread>
 
@@ -140,25 +141,52 @@ std::string parse_ale(const char* text, size_t len, const std::unordered_map<std
   long nset = (long)sets.size();
   for (long i = 1; i <= nset; ++i) if (!sets.count(i)) return "set ids are not contiguous";
   long G = nset + 1;
-  std::vector<std::vector<char>> mem(nset + 1);
-  for (long i = 1; i <= nset; ++i) {
-    mem[i].assign(nset + 1, 0);
-    for (long x : sets[i]) { if (x < 1 || x > nset) return "bad set member"; mem[i][x] = 1; }
-  }
+  // Complements are found through an additive hash of the members (a random
+  // 64-bit key per leaf): the complement of set i hashes to (sum over all
+  // leaves) - h(i).  Every candidate is then checked exactly (sizes add up to
+  // the leaves, no common member), and pairs come out in the (i, j) order of
+  // the reference's double loop.
+  auto key = [](long x) {
+    uint64_t z = (uint64_t)x * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  std::vector<uint64_t> h(nset + 1, 0);
+  for (long i = 1; i <= nset; ++i)
+    for (long x : sets[i]) {
+      if (x < 1 || x > nset) return "bad set member";
+      h[i] += key(x);
+    }
+  uint64_t all = 0;
+  for (auto& kv : leafclade) all += key(kv.first);
+  std::unordered_map<uint64_t, std::vector<long>> by_hash;
+  by_hash.reserve((size_t)nset * 2);
+  for (long i = 1; i <= nset; ++i) by_hash[h[i]].push_back(i);
+  std::vector<char> mark(nset + 1, 0);
   std::vector<D> root;
   double N = 0;
   for (long i = 1; i <= nset; ++i) {
-    long si = (long)sets[i].size();
-    for (long j = i + 1; j <= nset; ++j) {
-      if (si + (long)sets[j].size() != n) continue;
+    auto it = by_hash.find(all - h[i]);
+    if (it == by_hash.end()) continue;
+    const long si = (long)sets[i].size();
+    bool marked = false;
+    for (long j : it->second) {
+      if (j <= i || si + (long)sets[j].size() != n) continue;
+      if (!marked) {
+        for (long x : sets[i]) mark[x] = 1;
+        marked = true;
+      }
       bool disjoint = true;
-      for (long x : sets[j]) if (mem[i][x]) { disjoint = false; break; }
+      for (long x : sets[j]) if (mark[x]) { disjoint = false; break; }
       if (!disjoint) continue;
       if (!bip.count(i) || !bip.count(j)) return "missing Bip count";
       if (bip[i] != bip[j]) return "complementary clades have different Bip counts";
       N += bip[i];
       root.push_back({i, j, bip[j]});
     }
+    if (marked)
+      for (long x : sets[i]) mark[x] = 0;
   }
   if (root.empty()) return "no split of the ubiquitous clade";
   bip[G] = N;
