@@ -1,0 +1,95 @@
+"""MixtureModel and ModelArray likelihoods, mirroring src/core.jl:66-79, and
+the sampling condition of a model (src/condition.jl:11-29).
+
+    condition(wm)                       log P(sampling condition)
+    MixtureModel(components, prior)     logpdf: ℓhood(Σ_i logsumexp_j (ℓ_ij + log p_j − c_j))
+    ModelArray(models)                  logpdf: Σ_i logpdf(models[i], ccds[i])
+
+Each component is evaluated over the whole batch by the batched device
+launches (the components share the species tree, so they share the
+device-resident batch); the per-family values are combined on the host.
+"""
+from __future__ import annotations
+
+import math
+from typing import Sequence
+
+import numpy as np
+
+from .ccd import CCD
+from .core import logpdf, logpdf_families
+from .model import NoCondition, NonExtinctCondition, RootCondition
+
+
+def _geompgf(p: float, s: float) -> float:
+    """pgf of the geometric number of lineages at the root (src/bdputil.jl:67)."""
+    return p * s / (1.0 - (1.0 - p) * s)
+
+
+def condition(wm) -> float:
+    """The model's sampling condition as a log-probability (src/condition.jl:11-29),
+    from the device slice tables (ε at the top of the root and its children)."""
+    cond = wm.s.condition
+    if isinstance(cond, NoCondition):
+        return 0.0
+    tabs = wm.tables()
+    eta = float(wm.node_params()[3])
+    root = next(d for d in wm.s.nodes if d.isroot())
+    eps_top = lambda i: float(tabs[i][0][-1])
+    er = _geompgf(eta, eps_top(root.id))
+    if isinstance(cond, NonExtinctCondition):
+        return math.log(1.0 - er)
+    if isinstance(cond, RootCondition):
+        f, g = root.children
+        p = 1.0 - _geompgf(eta, eps_top(f)) - _geompgf(eta, eps_top(g)) + er
+        return math.log(p) if p > 0.0 else -math.inf
+    raise ValueError(f"condition {type(cond).__name__} is not supported")
+
+
+def _logsumexp_rows(a: np.ndarray) -> np.ndarray:
+    m = np.max(a, axis=1)
+    finite = np.isfinite(m)
+    out = np.full(a.shape[0], -np.inf)
+    mm = m[finite][:, None]
+    out[finite] = m[finite] + np.log(np.sum(np.exp(a[finite] - mm), axis=1))
+    return out
+
+
+class MixtureModel:
+    """A finite mixture of WhaleModels over families (Distributions.MixtureModel
+    with WhaleModel components, src/core.jl:66-76): each family comes from
+    component j with prior probability prior[j]."""
+
+    def __init__(self, components: Sequence, prior: Sequence[float]):
+        if len(components) != len(prior) or not components:
+            raise ValueError("one prior weight per component")
+        self.components = list(components)
+        self.prior = np.asarray(prior, dtype=np.float64)
+
+    def family_matrix(self, ccds, mode: str = "fp64") -> np.ndarray:
+        """ℓ[i, j] = logpdf(component j, family i) + log prior[j] − condition(component j)."""
+        cols = []
+        for wm, pj in zip(self.components, self.prior):
+            fam = logpdf_families(wm, ccds, mode=mode)
+            cols.append(fam + math.log(pj) - condition(wm))
+        return np.stack(cols, axis=1)
+
+    def logpdf(self, ccds, mode: str = "fp64") -> float:
+        tot = float(np.sum(_logsumexp_rows(self.family_matrix(ccds, mode))))
+        return tot if math.isfinite(tot) else -math.inf
+
+
+class ModelArray:
+    """One WhaleModel per family (src/model.jl:257-261, src/core.jl:78-79):
+    logpdf is the sum of the families' unconditioned log-likelihoods."""
+
+    def __init__(self, models: Sequence):
+        self.models = list(models)
+
+    def __len__(self):
+        return len(self.models)
+
+    def logpdf(self, ccds: Sequence[CCD], mode: str = "fp64") -> float:
+        if len(ccds) != len(self.models):
+            raise ValueError("one model per family")
+        return float(sum(logpdf(m, x, mode=mode) for m, x in zip(self.models, ccds)))
