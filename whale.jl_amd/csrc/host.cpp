@@ -422,6 +422,8 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
     }
     {
       uint32_t nrec = (uint32_t)F.rec_ops.size() - F.it_rec[sd.item0];
+      sd.rec0 = F.it_rec[sd.item0];
+      sd.n_recs = nrec;
       if (is_root) F.root_recs = nrec;
       else F.max_recs = std::max(F.max_recs, nrec);
     }
@@ -512,7 +514,12 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
     d.ibar0 = ibar;
     P.fam_seg0[f] = seg_base;
     P.fam_nseg[f] = (uint32_t)F.seg.size();
-    for (auto s : F.stage) { s.item0 += item_base; s.seg0 += seg_base; P.stage.push_back(s); }
+    for (auto s : F.stage) {
+      s.item0 += item_base;
+      s.seg0 += seg_base;
+      s.rec0 += rec_base;
+      P.stage.push_back(s);
+    }
     for (auto& s : F.seg) P.seg.push_back(s);
     for (size_t i = 0; i < F.it_seg.size(); ++i) {
       P.it_seg.push_back(F.it_seg[i] + seg_base);

@@ -580,8 +580,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
     for (uint32_t s = 0; s < F.n_stages; ++s) {
       const StageDev S = B.stage[F.stage0 + s];
       const uint32_t n = S.n_items, i0 = S.item0;
-      const uint32_t rb = B.it_rec[i0];
-      const uint32_t nrec = B.it_rec[i0 + n] - rb;
+      const uint32_t rb = S.rec0, nrec = S.n_recs;
       const FwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
       Real* const bA = T.bA;
       Real* const bB = T.bB;
@@ -1427,8 +1426,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
     for (int s = (int)F.n_stages - 1; s >= 0; --s) {
       const StageDev S = B.stage[F.stage0 + s];
       const uint32_t n = S.n_items, i0 = S.item0;
-      const uint32_t rb = B.it_rec[i0];
-      const uint32_t nrec = B.it_rec[i0 + n] - rb;
+      const uint32_t rb = S.rec0, nrec = S.n_recs;
       const BwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
       Real* bA = T.bA;  // d logL / d ℓ, current row
       Real* bB = T.bB;  // next row (and ī sums at row 0)

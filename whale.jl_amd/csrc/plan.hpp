@@ -64,7 +64,9 @@ struct StageDev {
   uint32_t r_max;        // largest R among its segments
   uint32_t seg0;         // absolute first segment
   uint32_t n_segs;
-  uint32_t pad[3];
+  uint32_t rec0;         // absolute first record (records of its items, owner by owner)
+  uint32_t n_recs;
+  uint32_t pad;
 };
 
 struct SegDev {
