@@ -73,3 +73,21 @@ def test_model_array_sums_single_families(w, cfg2):
     ref = sum(om.logpdf(oc, orates(comps[i % 3]), per_family=True)[i] for i in range(n))
     print("model array", got, ref)
     assert abs(got - ref) <= 1e-12 * abs(ref)
+
+
+def test_nowhere_extinct_batch_value(w):
+    """The batch value with NowhereExtinctCondition: Σ ℓ_i − N·c, c from the
+    tree's pgf on the host (the per-family values are unconditioned)."""
+    from whale_jl_amd import synth
+    wl = synth.workload("config1", n_families=10, nthreads=4)
+    m = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt, condition=w.NowhereExtinctCondition())
+    c = w.ccds_from_texts(wl.texts, m)
+    fam = w.logpdf_families(m, c)
+    om = OracleModel(wl.newick, [], wl.dt)
+    ref_fam = om.logpdf(om.ccds(wl.texts), orates(wl.rates), per_family=True)
+    assert np.max(np.abs(fam - ref_fam) / np.abs(ref_fam)) <= 1e-12
+    cond = w.condition(m)
+    assert np.isfinite(cond) and cond < 0
+    assert abs(w.logpdf(m, c) - (fam.sum() - len(fam) * cond)) <= 1e-12 * abs(fam.sum())
+    with pytest.raises(NotImplementedError):
+        w.gradient(m, c)

@@ -45,7 +45,13 @@ def _eval(wm, ccds, mode, launch, keep):
     if keep:
         for i, c in enumerate(ccds):
             c.ell = _split_ell(wm, c, b.ell(i))
-    return tot.value, fam
+    total = tot.value
+    if getattr(wm.s.condition, "host", False):
+        # a condition applied on the host: the device returned Σ ℓ_i
+        from .mixture import condition
+        total = total - len(fam) * condition(wm)
+        total = total if np.isfinite(total) else -np.inf
+    return total, fam
 
 
 def _split_ell(wm, ccd: CCD, flat: np.ndarray):
@@ -105,6 +111,8 @@ def gradient(wm, ccds, mode: str = "fp64", launch: str = "batched"):
     vector (DLWGD: [λ; μ; q; η], log-scale λ/μ; ConstantDLWGD: [λ, μ, q; η])."""
     if isinstance(ccds, CCD):
         ccds = _single_set(ccds)
+    if getattr(wm.s.condition, "host", False):
+        raise NotImplementedError(f"no gradient with {type(wm.s.condition).__name__}")
     h = wm._sync()
     b = device_batch(wm, ccds)
     n, nw = wm.s.n, wm.s.n_wgd

@@ -18,7 +18,7 @@ import numpy as np
 
 from .ccd import CCD
 from .core import logpdf, logpdf_families
-from .model import NoCondition, NonExtinctCondition, RootCondition
+from .model import NoCondition, NonExtinctCondition, NowhereExtinctCondition, RootCondition
 
 
 def _geompgf(p: float, s: float) -> float:
@@ -26,12 +26,59 @@ def _geompgf(p: float, s: float) -> float:
     return p * s / (1.0 - (1.0 - p) * s)
 
 
+def _bdp_pgf(lam: float, mu: float, t: float, s: float) -> float:
+    """pgf of the linear birth-death process over time t (src/bdputil.jl:52-59)."""
+    if abs(lam - mu) <= 1e-6:
+        return (1.0 - (lam * t - 1.0) * (s - 1.0)) / (1.0 - lam * t * (s - 1.0))
+    rho = math.exp((mu - lam) * t)
+    return (rho * (lam * s - mu) - mu * (s - 1.0)) / (rho * (lam * s - mu) - lam * (s - 1.0))
+
+
+def _wgdpgf(q: float, s: float) -> float:
+    return s * (1.0 - q + s * q)  # src/bdputil.jl:73
+
+
+def nowhere_extinct(wm) -> float:
+    """log P(no leaf family is empty) (src/condition.jl:31-36): the tree's
+    joint pgf at every all-binary argument (src/bdputil.jl:107-119), with the
+    inclusion–exclusion signs (src/bdputil.jl:121-130); entries in the
+    reference's order (first child's index fastest)."""
+    lam, mu, q, eta, _, _ = wm.node_params()
+    nodes = wm.s.nodes
+
+    def walk(d):
+        if d.isroot():
+            f = lambda x: _geompgf(eta, x)
+        else:
+            lam_e, mu_e, t = float(lam[d.id - 1]), float(mu[d.id - 1]), float(d.distance)
+            f = lambda x: _bdp_pgf(lam_e, mu_e, t, x)
+        if d.isleaf():
+            return [f(0.0), f(1.0)], [1.0, -1.0]
+        xs, sg = [1.0], [1.0]
+        for c in d.children:
+            v, s = walk(nodes[c - 1])
+            xs = [a * b for b in v for a in xs]
+            sg = [a * b for b in s for a in sg]
+        if d.wgd:
+            qq = float(q[d.wgdid - 1])
+            return [f(_wgdpgf(qq, x)) for x in xs], sg
+        return [f(x) for x in xs], sg
+
+    root = next(d for d in nodes if d.isroot())
+    p, sg = walk(root)
+    c = 1.0 - sum(a * b for a, b in zip(p[:-1], sg[:-1]))
+    return math.log(c) if 0.0 < c < 1.0 else -math.inf
+
+
 def condition(wm) -> float:
-    """The model's sampling condition as a log-probability (src/condition.jl:11-29),
-    from the device slice tables (ε at the top of the root and its children)."""
+    """The model's sampling condition as a log-probability (src/condition.jl:11-36):
+    Root and NonExtinct from the device slice tables (ε at the top of the root
+    and its children), NowhereExtinct from the tree's pgf on the host."""
     cond = wm.s.condition
     if isinstance(cond, NoCondition):
         return 0.0
+    if isinstance(cond, NowhereExtinctCondition):
+        return nowhere_extinct(wm)
     tabs = wm.tables()
     eta = float(wm.node_params()[3])
     root = next(d for d in wm.s.nodes if d.isroot())

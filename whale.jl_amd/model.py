@@ -39,6 +39,16 @@ class NoCondition(SamplingCondition):
     code = _lib.COND_NONE
 
 
+class NowhereExtinctCondition(SamplingCondition):
+    """Non-extinction at every leaf (src/condition.jl:5-9,31-36): inclusion–
+    exclusion over the 2^L evaluations of the tree's pgf at all-binary
+    arguments (src/bdputil.jl:107-130), so for trees with few leaves.  The
+    device evaluates the unconditioned values; the condition is applied on
+    the host (no gradient)."""
+    code = _lib.COND_NONE
+    host = True
+
+
 # ---------------------------------------------------------------- rates
 @dataclass
 class DLWGD:
