@@ -325,6 +325,91 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
       }
     }
   }
+  // A stage runs max R rows for all its segments, so the rows of a family are
+  // the sum of its stages' longest branches.  Stages by height put long
+  // branches of different lineages in different stages; a segment may run in
+  // any stage after its children's and before its parent's.  A local search
+  // moves segments (longest first) to the stage that lowers the modelled cost
+  // Σ (max R − 1 + 3) · (2 + ⌈items/64⌉ + ⌈records/64⌉) — rows times the
+  // 64-lane chunks of one row, plus a stage's set-up — without raising the
+  // family's largest stage (its LDS need) or putting clades twice in a stage.
+  if (stages.size() > 1) {
+    const int K = (int)stages.size();
+    std::vector<int> stg(m.n, -1);
+    std::vector<int64_t> nrec_of(m.n, 0);
+    std::vector<std::vector<int>> mem(K);
+    std::vector<int64_t> ni(K, 0), nr(K, 0);
+    for (int k = 0; k < K; ++k)
+      for (int e : stages[k]) {
+        stg[e] = k;
+        mem[k].push_back(e);
+        for (int32_t c : comp[e]) nrec_of[e] += soff[c + 1] - soff[c];
+        ni[k] += (int64_t)comp[e].size();
+        nr[k] += nrec_of[e];
+      }
+    int64_t cap_i = 0, cap_r = 0;
+    for (int k = 0; k < K; ++k) cap_i = std::max(cap_i, ni[k]), cap_r = std::max(cap_r, nr[k]);
+    auto rmax_of = [&](int k, int skip) {
+      int r = 0;
+      for (int e : mem[k]) if (e != skip) r = std::max(r, m.nodes[e].R);
+      return r;
+    };
+    auto cost = [](int rmax, int64_t items, int64_t recs) -> int64_t {
+      if (items == 0) return 0;
+      return (int64_t)(rmax - 1 + 3) * (2 + (items + 63) / 64 + (recs + 63) / 64);
+    };
+    std::vector<int> rmx(K);
+    for (int k = 0; k < K; ++k) rmx[k] = rmax_of(k, -1);
+    std::vector<int> by_r(segnodes);
+    std::stable_sort(by_r.begin(), by_r.end(), [&](int a, int b) { return m.nodes[a].R > m.nodes[b].R; });
+    std::vector<int> mark(nc, -1);
+    for (int pass = 0; pass < 8; ++pass) {
+      bool moved = false;
+      for (int e : by_r) {
+        const NodeDev& nd = m.nodes[e];
+        const int a = stg[e];
+        int lo = 0, hi = K - 1;
+        for (int ch : {nd.child0, nd.child1})
+          if (ch >= 0 && stg[ch] >= 0) lo = std::max(lo, stg[ch] + 1);
+        if (nd.parent >= 0 && nd.parent != m.root) hi = stg[nd.parent] - 1;
+        const int64_t ce = (int64_t)comp[e].size(), re = nrec_of[e];
+        const int ra = rmax_of(a, e);
+        const int64_t drop = cost(ra, ni[a] - ce, nr[a] - re) - cost(rmx[a], ni[a], nr[a]);
+        int best = a;
+        int64_t gain = 0;
+        for (int b = lo; b <= hi; ++b) {
+          if (b == a || ni[b] + ce > cap_i || nr[b] + re > cap_r) continue;
+          const int64_t d = drop + cost(std::max(rmx[b], nd.R), ni[b] + ce, nr[b] + re) - cost(rmx[b], ni[b], nr[b]);
+          if (d >= gain) continue;
+          // no clade twice in one stage (MUL trees)
+          for (int32_t c : comp[e]) mark[c] = e;
+          bool clash = false;
+          for (int x : mem[b]) {
+            for (int32_t c : comp[x]) if (mark[c] == e) { clash = true; break; }
+            if (clash) break;
+          }
+          for (int32_t c : comp[e]) mark[c] = -1;
+          if (clash) continue;
+          gain = d;
+          best = b;
+        }
+        if (best == a) continue;
+        mem[a].erase(std::find(mem[a].begin(), mem[a].end(), e));
+        mem[best].push_back(e);
+        ni[a] -= ce, nr[a] -= re, ni[best] += ce, nr[best] += re;
+        rmx[a] = ra;
+        rmx[best] = std::max(rmx[best], nd.R);
+        stg[e] = best;
+        moved = true;
+      }
+      if (!moved) break;
+    }
+    // non-empty stages in order, segments in the species tree's order
+    std::vector<std::vector<int>> out(K);
+    for (int e : segnodes) out[stg[e]].push_back(e);
+    stages.clear();
+    for (auto& st : out) if (!st.empty()) stages.push_back(std::move(st));
+  }
   // segment records, ℓ and ī offsets
   auto nkids = [&](int e) { const NodeDev& d = m.nodes[e]; return (d.child0 >= 0) + (d.child1 >= 0); };
   uint64_t ell = 0, ibar = 0;
