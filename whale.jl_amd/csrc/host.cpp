@@ -334,13 +334,17 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
   // 64-lane chunks of one row, plus a stage's set-up — without raising the
   // family's largest stage (its LDS need) or putting clades twice in a stage.
   if (stages.size() > 1) {
-    const int K = (int)stages.size();
+    // the height stages take every kGap-th slot; the slots between are empty
+    // at first, so a segment can also run between two height stages
+    constexpr int kGap = 2;
+    const int K = kGap * ((int)stages.size() - 1) + 1;
     std::vector<int> stg(m.n, -1);
     std::vector<int64_t> nrec_of(m.n, 0);
     std::vector<std::vector<int>> mem(K);
     std::vector<int64_t> ni(K, 0), nr(K, 0);
-    for (int k = 0; k < K; ++k)
-      for (int e : stages[k]) {
+    for (int h = 0; h < (int)stages.size(); ++h)
+      for (int e : stages[h]) {
+        const int k = kGap * h;
         stg[e] = k;
         mem[k].push_back(e);
         for (int32_t c : comp[e]) nrec_of[e] += soff[c + 1] - soff[c];
