@@ -114,6 +114,15 @@ int whale_model_tables(whale_model* m, double* eps, double* phi, double* psi);
  * to species index species_ids[k] (spmap of src/ccd.jl:128). */
 int whale_ccd_parse(const char* text, size_t len, const char* const* species_names,
                     const int32_t* species_ids, int32_t n_names, whale_ccd** out);
+/* read_ale over many files (src/ccd.jl:126-137 maps CCD(...) over them one by
+ * one): whale_ccd_parse of texts[i] (lens[i] bytes) for every i < n, on up to
+ * n_threads host threads (<= 0: the hardware threads, at most 16).  On
+ * success out[i] holds family i.  If a text does not parse, no handle is
+ * returned: *bad (when not NULL) is the lowest such index, and the error
+ * message names it. */
+int whale_ccd_parse_many(const char* const* texts, const size_t* lens, int32_t n,
+                         const char* const* species_names, const int32_t* species_ids, int32_t n_names,
+                         int32_t n_threads, whale_ccd** out, int32_t* bad);
 int whale_ccd_shape(const whale_ccd* c, int32_t* n_clades, int32_t* n_splits);
 /* Clades in the reference's order (by size, then ALE set id; 0-based ids).
  * clade_size[n_clades], leaf_species[n_clades] (-1 for non-leaf clades),

@@ -95,6 +95,46 @@ def test_parse_errors_raise(w):
         w.CCD.from_text(bad, m.s.spmap)
 
 
+def test_parse_many_matches_one_by_one(w):
+    """whale_ccd_parse_many, at any thread count, returns the families that
+    whale_ccd_parse returns one by one, in order.  When texts do not parse,
+    the error names the lowest such file and no family is returned."""
+    from whale_jl_amd import synth
+    from whale_jl_amd.ccd import parse_many
+    wl = synth.workload("config4", n_families=40)
+    m = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt)
+    texts = list(wl.texts) + [wl.texts[3]]  # and a repeated family
+    names = [f"fam{i}" for i in range(len(texts))]
+    one = [w.CCD.from_text(t, m.s.spmap) for t in texts]
+    for nt in (0, 1, 3, 64):
+        many = parse_many(texts, m.s.spmap, names, n_threads=nt)
+        assert [c.fname for c in many] == names
+        for a, b in zip(many, one, strict=True):
+            for k in ("size", "leaf_species", "split_off", "g1", "g2", "p"):
+                assert np.array_equal(getattr(a, k), getattr(b, k))
+    assert parse_many([], m.s.spmap, []) == []
+    bad = list(texts)
+    bad[19] = bad[19].replace("#Bip_counts", "#Bip_count")
+    bad[7] = bad[7].replace("#Dip_counts", "#Dip_count")
+    with pytest.raises(ValueError, match=r"read_ale fam7: family 7: ALE section missing: Dip_counts"):
+        parse_many(bad, m.s.spmap, names, n_threads=4)
+
+
+def test_read_ale_directory_matches_one_by_one(w):
+    """read_ale on a directory (files sorted by name, parsed together)
+    equals from_text on each file."""
+    m = kat_structure(w)
+    d = os.path.join(GOLDEN, "example-1")
+    files = sorted(os.listdir(d))
+    got = w.read_ale(d, m)
+    assert [c.fname for c in got] == files
+    for c, f in zip(got, files, strict=True):
+        with open(os.path.join(d, f)) as fh:
+            ref = w.CCD.from_text(fh.read(), m.s.spmap)
+        for k in ("size", "leaf_species", "split_off", "g1", "g2", "p"):
+            assert np.array_equal(getattr(c, k), getattr(ref, k))
+
+
 def test_workload_is_deterministic(w):
     from whale_jl_amd import synth
     a = synth.workload("config4", n_families=30, nthreads=3)

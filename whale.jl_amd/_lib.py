@@ -54,6 +54,9 @@ def lib():
         L.whale_model_table_offsets.argtypes = [vp, i32p]
         L.whale_model_tables.argtypes = [vp, dblp, dblp, dblp]
         L.whale_ccd_parse.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(C.c_char_p), i32p, C.c_int32, C.POINTER(vp)]
+        L.whale_ccd_parse_many.argtypes = [C.POINTER(C.c_char_p), C.POINTER(C.c_size_t), C.c_int32,
+                                           C.POINTER(C.c_char_p), i32p, C.c_int32, C.c_int32, C.POINTER(vp),
+                                           C.POINTER(C.c_int32)]
         L.whale_ccd_shape.argtypes = [vp, i32p, i32p]
         L.whale_ccd_export.argtypes = [vp, i32p, i32p, i32p, i32p, i32p, dblp]
         L.whale_ccd_destroy.argtypes = [vp]

@@ -1,5 +1,5 @@
 """CCDs and read_ale, mirroring src/ccd.jl.  Parsing is done by the native
-library (whale_ccd_parse); a CCD holds the flattened clades in the
+library (whale_ccd_parse, whale_ccd_parse_many); a CCD holds the flattened clades in the
 reference's order (sorted by size then ALE set id; leaves first, the
 ubiquitous clade last), with 0-based clade ids."""
 from __future__ import annotations
@@ -40,27 +40,66 @@ class CCD:
 
     @staticmethod
     def from_text(text: str, spmap, fname: str = "") -> "CCD":
-        names = list(spmap.keys())
-        ids = np.array([spmap[k] - 1 for k in names], np.int32)
-        arr = (C.c_char_p * len(names))(*[k.encode() for k in names])
+        names, ids, arr = _species_args(spmap)
         raw = text.encode()
         h = C.c_void_p()
         check(_lib.lib().whale_ccd_parse(raw, len(raw), arr, ptr(ids, i32p), len(names), C.byref(h)),
               f"read_ale {fname}")
-        try:
-            nc, ns = C.c_int32(), C.c_int32()
-            check(_lib.lib().whale_ccd_shape(h, C.byref(nc), C.byref(ns)), "whale_ccd_shape")
-            size = np.zeros(nc.value, np.int32)
-            sp = np.zeros(nc.value, np.int32)
-            off = np.zeros(nc.value + 1, np.int32)
-            g1 = np.zeros(ns.value, np.int32)
-            g2 = np.zeros(ns.value, np.int32)
-            p = np.zeros(ns.value)
-            check(_lib.lib().whale_ccd_export(h, ptr(size, i32p), ptr(sp, i32p), ptr(off, i32p), ptr(g1, i32p),
-                                              ptr(g2, i32p), ptr(p, dblp)), "whale_ccd_export")
-        finally:
-            _lib.lib().whale_ccd_destroy(h)
-        return CCD(size, sp, off, g1, g2, p, fname)
+        return _take(h, fname)
+
+
+def _species_args(spmap):
+    names = list(spmap.keys())
+    ids = np.array([spmap[k] - 1 for k in names], np.int32)
+    arr = (C.c_char_p * len(names))(*[k.encode() for k in names])
+    return names, ids, arr
+
+
+def _take(h, fname: str) -> CCD:
+    """Export a parsed family (whale_ccd_export) and free its handle."""
+    try:
+        nc, ns = C.c_int32(), C.c_int32()
+        check(_lib.lib().whale_ccd_shape(h, C.byref(nc), C.byref(ns)), "whale_ccd_shape")
+        size = np.zeros(nc.value, np.int32)
+        sp = np.zeros(nc.value, np.int32)
+        off = np.zeros(nc.value + 1, np.int32)
+        g1 = np.zeros(ns.value, np.int32)
+        g2 = np.zeros(ns.value, np.int32)
+        p = np.zeros(ns.value)
+        check(_lib.lib().whale_ccd_export(h, ptr(size, i32p), ptr(sp, i32p), ptr(off, i32p), ptr(g1, i32p),
+                                          ptr(g2, i32p), ptr(p, dblp)), "whale_ccd_export")
+    finally:
+        _lib.lib().whale_ccd_destroy(h)
+    return CCD(size, sp, off, g1, g2, p, fname)
+
+
+def parse_many(texts: Sequence, spmap, fnames: Sequence[str], n_threads: int = 0) -> List[CCD]:
+    """ALE texts (str or bytes) to CCDs, parsed on host threads
+    (whale_ccd_parse_many); the error of a text that does not parse names
+    its file."""
+    n = len(texts)
+    if n == 0:
+        return []
+    raw = [t if isinstance(t, bytes) else t.encode() for t in texts]
+    names, ids, arr = _species_args(spmap)
+    tarr = (C.c_char_p * n)(*raw)
+    lens = (C.c_size_t * n)(*[len(b) for b in raw])
+    hs = (C.c_void_p * n)()
+    bad = C.c_int32(-1)
+    rc = _lib.lib().whale_ccd_parse_many(tarr, lens, n, arr, ptr(ids, i32p), len(names), n_threads, hs,
+                                         C.byref(bad))
+    if rc != _lib.WHALE_OK:
+        check(rc, f"read_ale {fnames[bad.value]}" if 0 <= bad.value < n else "read_ale")
+    out = []
+    try:
+        for i in range(n):
+            h, hs[i] = hs[i], None
+            out.append(_take(h, fnames[i]))
+    finally:
+        for i in range(len(out), n):  # handles left by a failed export
+            if hs[i]:
+                _lib.lib().whale_ccd_destroy(hs[i])
+    return out
 
 
 class CCDSet(list):
@@ -85,18 +124,16 @@ def read_ale(path: str, wm) -> CCDSet:
         else:
             files = [os.path.join(path, f) for f in sorted(os.listdir(path))]
         files = [f for f in files if not f.startswith("#")]
-    out = CCDSet()
+    texts = []
     for f in files:
-        with open(f) as fh:
-            out.append(CCD.from_text(fh.read(), spmap, os.path.basename(f)))
-    return out
+        with open(f) as fh:  # text mode: line endings normalised
+            texts.append(fh.read())
+    return CCDSet(parse_many(texts, spmap, [os.path.basename(f) for f in files]))
 
 
 def ccds_from_texts(texts: Sequence[str], wm, names: Optional[Sequence[str]] = None) -> CCDSet:
-    out = CCDSet()
-    for i, t in enumerate(texts):
-        out.append(CCD.from_text(t, wm.s.spmap, names[i] if names else f"family{i}"))
-    return out
+    fnames = list(names) if names else [f"family{i}" for i in range(len(texts))]
+    return CCDSet(parse_many(texts, wm.s.spmap, fnames))
 
 
 class DeviceBatch:

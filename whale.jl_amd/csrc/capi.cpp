@@ -480,6 +480,41 @@ int whale_ccd_parse(const char* text, size_t len, const char* const* names, cons
   return WHALE_OK;
 }
 
+int whale_ccd_parse_many(const char* const* texts, const size_t* lens, int32_t n, const char* const* names,
+                         const int32_t* ids, int32_t n_names, int32_t n_threads, whale_ccd** out, int32_t* bad) {
+  if (bad) *bad = -1;
+  if (n < 0 || (n > 0 && (!texts || !lens || !out)) || (n_names > 0 && (!names || !ids)))
+    return fail(WHALE_EINVAL, "whale_ccd_parse_many: NULL argument");
+  std::unordered_map<std::string, int> sp;
+  for (int k = 0; k < n_names; ++k) sp[names[k]] = ids[k];
+  std::vector<std::unique_ptr<whale_ccd>> res(n);
+  std::vector<std::string> err(n);
+  const unsigned hw = std::min(std::max(1u, std::thread::hardware_concurrency()), 16u);
+  const int nt = std::max(1, std::min(n_threads > 0 ? (int)n_threads : (int)hw, (int)n));
+  auto work = [&](int w) {
+    for (int i = w; i < n; i += nt) {
+      if (!texts[i]) {
+        err[i] = "NULL text";
+        continue;
+      }
+      auto c = std::make_unique<whale_ccd>();
+      err[i] = parse_ale(texts[i], lens[i], sp, c->c);
+      if (err[i].empty()) res[i] = std::move(c);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < nt; ++w) th.emplace_back(work, w);
+  work(0);
+  for (auto& t : th) t.join();
+  for (int i = 0; i < n; ++i)
+    if (!res[i]) {
+      if (bad) *bad = i;
+      return fail(WHALE_EPARSE, "family " + std::to_string(i) + ": " + err[i]);
+    }
+  for (int i = 0; i < n; ++i) out[i] = res[i].release();
+  return WHALE_OK;
+}
+
 int whale_ccd_shape(const whale_ccd* c, int32_t* nc, int32_t* ns) {
   if (!c || !nc || !ns) return fail(WHALE_EINVAL, "NULL argument");
   *nc = (int32_t)c->c.size.size();
