@@ -527,9 +527,18 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
   StageDev rsd = emit_stage({m.root}, true);
   if (!F.err.empty()) return;
   (void)rsd;
-  // root size levels
+  // root levels: runs of clades in size order of which none is a part of
+  // another, so a level reads only finished parts.  Clades of one size never
+  // are; a run goes on past a size until a clade has a part inside it.
   F.lvl.push_back(0);
-  for (int c = 1; c < nc; ++c) if (csize[c] != csize[c - 1]) F.lvl.push_back((uint32_t)c);
+  for (int c = 1, start = 0; c < nc; ++c) {
+    int32_t dep = -1;
+    for (int64_t k = soff[c]; k < soff[c + 1]; ++k) dep = std::max(dep, std::max(g1[k], g2[k]));
+    if (dep >= start) {
+      F.lvl.push_back((uint32_t)c);
+      start = c;
+    }
+  }
   F.lvl.push_back((uint32_t)nc);
   // workload counts (the reference's cell and split-term counts)
   for (auto& s : F.seg) {
