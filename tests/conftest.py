@@ -22,6 +22,17 @@ KAT1 = -60.96367806571888
 KAT2 = -592.0185620440255
 
 
+def unpack_rates(rates, v):
+    """Rates of the same kind as `rates` from a parameter vector laid out as
+    rates.pack() lays it out."""
+    v = [float(x) for x in v]
+    nq = len(rates.q)
+    if isinstance(rates.λ, (int, float)):
+        return type(rates)(λ=v[0], μ=v[1], q=v[2:2 + nq], p=rates.p, η=v[-1])
+    nl = len(rates.λ)
+    return type(rates)(λ=v[:nl], μ=v[nl:2 * nl], q=v[2 * nl:2 * nl + nq], p=rates.p, η=v[-1])
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
 

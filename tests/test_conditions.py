@@ -1,12 +1,16 @@
 """NowhereExtinctCondition (src/condition.jl:31-36) on the host: pinned by the
 reference's own monotonicity check (test/runtests.jl:55-64) and by a direct
 sum over lineage counts with the linear BDP transition probabilities
-(src/bdputil.jl:30-36), which shares no code with the pgf route.  No GPU."""
+(src/bdputil.jl:30-36), which shares no code with the pgf route.  The
+gradient (complex-step) is pinned by central differences of the value.
+No GPU."""
 import math
+
+import numpy as np
 
 import pytest
 
-from conftest import EXTREE, KAT_INSERTS
+from conftest import EXTREE, KAT_INSERTS, unpack_rates
 
 
 def kat_tree(w):
@@ -85,3 +89,38 @@ def test_nowhere_extinct_matches_count_sum(w, rates):
     ref = _brute_nowhere(m)
     print(rates, got, ref)
     assert abs(got - ref) <= 1e-9
+
+
+def _check_grad_vs_central_differences(w, m, h=1e-6, tol=1e-6):
+    from whale_jl_amd.mixture import nowhere_extinct, nowhere_extinct_grad
+    g = m.chain_gradient(*nowhere_extinct_grad(m))
+    th = m.rates.pack()
+    fd = np.zeros_like(th)
+    for i in range(len(th)):
+        up, dn = th.copy(), th.copy()
+        up[i] += h
+        dn[i] -= h
+        fd[i] = (nowhere_extinct(m(unpack_rates(m.rates, up))) - nowhere_extinct(m(unpack_rates(m.rates, dn)))) / (2 * h)
+    assert np.max(np.abs(g - fd)) <= tol * max(1.0, np.max(np.abs(fd))), (g, fd)
+    # zero only where c does not depend on the entry (the root's λ, μ; μ of a
+    # λ = μ branch, whose pgf form has λ alone)
+    assert np.count_nonzero(g) >= len(g) - 3
+
+
+def test_nowhere_extinct_gradient_constant_rates_with_wgds(w):
+    """Complex-step ∂ log c against central differences of the value,
+    ConstantDLWGD on a tree with two WGD nodes: [λ, μ, q1, q2, η]."""
+    m = w.WhaleModel(w.ConstantDLWGD(λ=0.3, μ=0.45, q=[0.2, 0.35], η=0.7), kat_tree(w), 0.05,
+                     condition=w.NowhereExtinctCondition())
+    _check_grad_vs_central_differences(w, m)
+
+
+def test_nowhere_extinct_gradient_branch_rates(w):
+    """The same with branch-wise log-scale rates, λ ≠ μ and λ = μ branches."""
+    t = w.readnw("((A:0.6,B:0.4):0.5,C:0.9);")
+    n = len(w.postwalk(t))
+    lam = [math.log(0.2 + 0.05 * i) for i in range(n)]
+    mu = [math.log(0.3 + 0.04 * i) for i in range(n)]
+    mu[1] = lam[1]  # the λ = μ form of the pgf (within 1e-6 under the steps)
+    m = w.WhaleModel(w.DLWGD(λ=lam, μ=mu, q=[], η=0.8), t, 0.05, condition=w.NowhereExtinctCondition())
+    _check_grad_vs_central_differences(w, m)

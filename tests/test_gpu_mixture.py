@@ -7,6 +7,7 @@ import math
 import numpy as np
 import pytest
 
+from conftest import unpack_rates
 from oracle.oracle import OracleModel, DLWGD as ODLWGD
 
 pytestmark = pytest.mark.gpu
@@ -75,9 +76,10 @@ def test_model_array_sums_single_families(w, cfg2):
     assert abs(got - ref) <= 1e-12 * abs(ref)
 
 
-def test_nowhere_extinct_batch_value(w):
+def test_nowhere_extinct_batch_value_and_gradient(w):
     """The batch value with NowhereExtinctCondition: Σ ℓ_i − N·c, c from the
-    tree's pgf on the host (the per-family values are unconditioned)."""
+    tree's pgf on the host (the per-family values are unconditioned), and its
+    gradient."""
     from whale_jl_amd import synth
     wl = synth.workload("config1", n_families=10, nthreads=4)
     m = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt, condition=w.NowhereExtinctCondition())
@@ -89,5 +91,18 @@ def test_nowhere_extinct_batch_value(w):
     cond = w.condition(m)
     assert np.isfinite(cond) and cond < 0
     assert abs(w.logpdf(m, c) - (fam.sum() - len(fam) * cond)) <= 1e-12 * abs(fam.sum())
-    with pytest.raises(NotImplementedError):
-        w.gradient(m, c)
+    # gradient: the device's unconditioned gradient minus N·∂c (complex step
+    # on the host), against central differences of the conditioned FP64 value
+    v, g, _ = w.gradient(m, c)
+    assert abs(v - w.logpdf(m, c)) <= 1e-12 * abs(v)
+    th = m.rates.pack()
+    fd = np.zeros_like(th)
+    h = 1e-5
+    for i in range(len(th)):
+        up, dn = th.copy(), th.copy()
+        up[i] += h
+        dn[i] -= h
+        fd[i] = (w.logpdf(m(unpack_rates(m.rates, up)), c) - w.logpdf(m(unpack_rates(m.rates, dn)), c)) / (2 * h)
+    err = np.max(np.abs(g - fd)) / np.max(np.abs(fd))
+    print("NowhereExtinct gradient vs central differences: rel err", err)
+    assert err <= 1e-5

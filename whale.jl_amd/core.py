@@ -111,8 +111,11 @@ def gradient(wm, ccds, mode: str = "fp64", launch: str = "batched"):
     vector (DLWGD: [λ; μ; q; η], log-scale λ/μ; ConstantDLWGD: [λ, μ, q; η])."""
     if isinstance(ccds, CCD):
         ccds = _single_set(ccds)
-    if getattr(wm.s.condition, "host", False):
-        raise NotImplementedError(f"no gradient with {type(wm.s.condition).__name__}")
+    host = getattr(wm.s.condition, "host", False)
+    if host:
+        from .model import NowhereExtinctCondition
+        if not isinstance(wm.s.condition, NowhereExtinctCondition):
+            raise NotImplementedError(f"no gradient with {type(wm.s.condition).__name__}")
     h = wm._sync()
     b = device_batch(wm, ccds)
     n, nw = wm.s.n, wm.s.n_wgd
@@ -124,8 +127,20 @@ def gradient(wm, ccds, mode: str = "fp64", launch: str = "batched"):
     check(_lib.lib().whale_logpdf_grad(b.h, _mode(mode), _launch(launch), ptr(fam, dblp), C.byref(tot),
                                        ptr(dl, dblp), ptr(dm, dblp), ptr(dq, dblp), C.byref(deta)),
           "whale_logpdf_grad")
-    g = wm.chain_gradient(dl, dm, dq, deta.value)
-    return tot.value, g, fam
+    total, dt = tot.value, deta.value
+    if host:
+        # a condition applied on the host: the device returned Σ ℓ_i and its
+        # gradient; subtract N·c and N·∂c (natural scale, per node)
+        from .mixture import nowhere_extinct, nowhere_extinct_grad
+        nf = len(fam)
+        cl, cm, cq, ce = nowhere_extinct_grad(wm)
+        total -= nf * nowhere_extinct(wm)
+        dl -= nf * cl
+        dm -= nf * cm
+        dq[:len(cq)] -= nf * cq
+        dt -= nf * ce
+    g = wm.chain_gradient(dl, dm, dq, dt)
+    return total, g, fam
 
 
 def _single_set(ccd: CCD) -> CCDSet:

@@ -21,36 +21,36 @@ from .core import logpdf, logpdf_families
 from .model import NoCondition, NonExtinctCondition, NowhereExtinctCondition, RootCondition
 
 
-def _geompgf(p: float, s: float) -> float:
+def _geompgf(p, s):
     """pgf of the geometric number of lineages at the root (src/bdputil.jl:67)."""
     return p * s / (1.0 - (1.0 - p) * s)
 
 
-def _bdp_pgf(lam: float, mu: float, t: float, s: float) -> float:
-    """pgf of the linear birth-death process over time t (src/bdputil.jl:52-59)."""
-    if abs(lam - mu) <= 1e-6:
+def _bdp_pgf(lam, mu, t: float, s):
+    """pgf of the linear birth-death process over time t (src/bdputil.jl:52-59);
+    real or complex arguments (the branch test uses the real parts)."""
+    if abs((lam - mu).real) <= 1e-6:
         return (1.0 - (lam * t - 1.0) * (s - 1.0)) / (1.0 - lam * t * (s - 1.0))
-    rho = math.exp((mu - lam) * t)
+    rho = np.exp((mu - lam) * t)
     return (rho * (lam * s - mu) - mu * (s - 1.0)) / (rho * (lam * s - mu) - lam * (s - 1.0))
 
 
-def _wgdpgf(q: float, s: float) -> float:
+def _wgdpgf(q, s):
     return s * (1.0 - q + s * q)  # src/bdputil.jl:73
 
 
-def nowhere_extinct(wm) -> float:
-    """log P(no leaf family is empty) (src/condition.jl:31-36): the tree's
-    joint pgf at every all-binary argument (src/bdputil.jl:107-119), with the
-    inclusion–exclusion signs (src/bdputil.jl:121-130); entries in the
-    reference's order (first child's index fastest)."""
-    lam, mu, q, eta, _, _ = wm.node_params()
-    nodes = wm.s.nodes
-
+def _nowhere_c(nodes, lam, mu, q, eta):
+    """P(no leaf family is empty) (src/condition.jl:31-36): one minus the
+    signed sum of the tree's joint pgf at every all-binary argument but all
+    ones (src/bdputil.jl:107-130), entries in the reference's order (first
+    child's index fastest).  lam, mu per node id - 1, q per WGD id - 1;
+    every operation is analytic, so complex arguments give complex-step
+    derivatives."""
     def walk(d):
         if d.isroot():
             f = lambda x: _geompgf(eta, x)
         else:
-            lam_e, mu_e, t = float(lam[d.id - 1]), float(mu[d.id - 1]), float(d.distance)
+            lam_e, mu_e, t = lam[d.id - 1], mu[d.id - 1], float(d.distance)
             f = lambda x: _bdp_pgf(lam_e, mu_e, t, x)
         if d.isleaf():
             return [f(0.0), f(1.0)], [1.0, -1.0]
@@ -60,14 +60,47 @@ def nowhere_extinct(wm) -> float:
             xs = [a * b for b in v for a in xs]
             sg = [a * b for b in s for a in sg]
         if d.wgd:
-            qq = float(q[d.wgdid - 1])
+            qq = q[d.wgdid - 1]
             return [f(_wgdpgf(qq, x)) for x in xs], sg
         return [f(x) for x in xs], sg
 
     root = next(d for d in nodes if d.isroot())
     p, sg = walk(root)
-    c = 1.0 - sum(a * b for a, b in zip(p[:-1], sg[:-1]))
+    return 1.0 - sum(a * b for a, b in zip(p[:-1], sg[:-1]))
+
+
+def nowhere_extinct(wm) -> float:
+    """log P(no leaf family is empty) (src/condition.jl:31-36)."""
+    lam, mu, q, eta, _, _ = wm.node_params()
+    c = float(np.real(_nowhere_c(wm.s.nodes, [float(x) for x in lam], [float(x) for x in mu],
+                                 [float(x) for x in q], float(eta))))
     return math.log(c) if 0.0 < c < 1.0 else -math.inf
+
+
+def nowhere_extinct_grad(wm):
+    """∂ log P(no leaf family is empty) / ∂(λ_e, μ_e per node, q per WGD, η),
+    natural scale: each parameter in turn gets an imaginary step ih, and
+    ∂c = Im c(θ + ih) / h (complex-step differentiation, Squire and Trapp,
+    SIAM Review 40, 1998), exact to rounding at h = 1e-30 since c is analytic
+    in every parameter."""
+    lam, mu, q, eta, _, _ = wm.node_params()
+    nodes = wm.s.nodes
+    base = [[complex(float(x)) for x in lam], [complex(float(x)) for x in mu], [complex(float(x)) for x in q],
+            [complex(float(eta))]]
+    c = float(np.real(_nowhere_c(nodes, base[0], base[1], base[2], base[3][0])))
+    if not 0.0 < c < 1.0:
+        raise ValueError("the NowhereExtinct condition is not a probability in (0, 1) at these parameters")
+    h = 1e-30
+    out = []
+    for k, vec in enumerate(base):
+        d = np.zeros(len(vec))
+        for i in range(len(vec)):
+            args = [list(v) for v in base]
+            args[k][i] += 1j * h
+            d[i] = np.imag(_nowhere_c(nodes, args[0], args[1], args[2], args[3][0])) / h / c
+        out.append(d)
+    dl, dm, dq, deta = out
+    return dl, dm, dq, float(deta[0])
 
 
 def condition(wm) -> float:
