@@ -118,8 +118,8 @@ def main():
                     help="also run one device-to-device copy of 256 MiB (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (families would be sharded across "
-              "GPUs with one all-reduce of the gradient; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (families would be sharded across "
+              "GPUs with one all-reduce of the gradient). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from __graft_entry__ import load_package

@@ -1,7 +1,8 @@
 """ctypes front for the CPU oracle (oracle/whale_oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, where it is the checker.  The product package
+cpu_baseline leg of bench.py, as the reference values the HIP path is compared
+with (and, in bench.py, the CPU baseline that is timed).  The product package
 never imports this module.
 
 The interface follows the reference's Julia API closely enough for tests to

@@ -2,8 +2,9 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This file is a plain CPU restatement of the
 // reference algorithm.  It is used by tests/, by __graft_entry__.smoke() and by
-// bench.py's cpu_baseline leg as the checker -- never by the product library,
-// which must not link, load or call anything under oracle/.
+// bench.py's cpu_baseline leg, for reference values and the timed CPU
+// baseline -- never by the product library, which must not link, load or call
+// anything under oracle/.
 //
 // Everything the path needs is restated here independently of the product:
 //   * a Newick reader and NewickTree-style insertnode! (halfway placement)
@@ -32,8 +33,9 @@
 //     @simd (reassociating) loop, so the last bits of the batch value may
 //     differ.
 //
-// Parity status of each function (how it is pinned) is listed in DESIGN.md's
-// oracle table; the tests live in tests/test_oracle_*.py.
+// Each function is pinned by tests/test_oracle.py: the reference's golden
+// log-likelihoods, closed forms of the birth-death pgf and its derivatives,
+// exact small cases, and central differences for the dual gradient.
 
 #include <algorithm>
 #include <array>
