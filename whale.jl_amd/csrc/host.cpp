@@ -330,9 +330,11 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
   // branches of different lineages in different stages; a segment may run in
   // any stage after its children's and before its parent's.  A local search
   // moves segments (longest first) to the stage that lowers the modelled cost
-  // Σ (max R − 1 + 3) · (2 + ⌈items/64⌉ + ⌈records/64⌉) — rows times the
-  // 64-lane chunks of one row, plus a stage's set-up — without raising the
-  // family's largest stage (its LDS need) or putting clades twice in a stage.
+  // Σ (max R − 1 + 3) · (2 + ⌈items/64⌉ + 2⌈records/64⌉) — rows times the
+  // 64-lane chunks of one row, record chunks weighing double (they gather two
+  // operands, and scatter twice in the reverse sweep), plus a stage's set-up —
+  // without raising the family's largest stage (its LDS need) or putting
+  // clades twice in a stage.
   if (stages.size() > 1) {
     // the height stages take every kGap-th slot; the slots between are empty
     // at first, so a segment can also run between two height stages
@@ -360,7 +362,7 @@ void plan_family(const ModelHost& m, int32_t nc, const int32_t* csize, const int
     };
     auto cost = [](int rmax, int64_t items, int64_t recs) -> int64_t {
       if (items == 0) return 0;
-      return (int64_t)(rmax - 1 + 3) * (2 + (items + 63) / 64 + (recs + 63) / 64);
+      return (int64_t)(rmax - 1 + 3) * (2 + (items + 63) / 64 + 2 * ((recs + 63) / 64));
     };
     std::vector<int> rmx(K);
     for (int k = 0; k < K; ++k) rmx[k] = rmax_of(k, -1);
