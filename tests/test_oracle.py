@@ -286,3 +286,173 @@ def test_logl_invariant_to_clade_relabelling():
     a = m.logpdf(m.ccds(texts), kat_rates(), per_family=True)
     b = m.logpdf(m.ccds([relabel_ale(t, i) for i, t in enumerate(texts)]), kat_rates(), per_family=True)
     assert np.all(np.abs(a - b) <= 1e-12 * np.abs(a))
+
+
+# ------------------------------------------------------------------ leaf sampling p
+@pytest.mark.parametrize("p", [0.0, 0.15, 0.6])
+def test_leaf_top_eps_is_pgf_at_p(p):
+    # ε at row 1 of a leaf is p (src/model.jl:170, src/rmodels.jl:14), so ε at
+    # the top of the leaf branch is the branch pgf at s = p and Π φ its slope there
+    tree = "(A:0.73,B:0.41);"
+    lam, mu = 0.9, 0.35
+    for dt in (0.05, 0.005):
+        m = OracleModel(tree, [], dt, minn=1)
+        nodes = {d["name"]: d for d in m.nodes()}
+        r = DLWGD([math.log(lam)] * 3, [math.log(mu)] * 3, [], 0.9, [p, 0.3])
+        e, phi, psi = m.tables(r, nodes["A"]["id"], nodes["A"]["k"] + 1)
+        f = lambda s: bdp_pgf(lam, mu, 0.73, s)
+        assert e[0] == p
+        assert abs(e[-1] - float(f(p))) <= 1e-13
+        assert abs(np.prod(phi[1:]) - float(mp.diff(f, p))) <= 1e-12
+
+
+@pytest.mark.parametrize("p", [0.1, 0.45])
+def test_two_paralogs_with_leaf_sampling_exact(p):
+    # with ε = p at the leaf, ℓ_A({A_1}) at the top is f_t'(p) and
+    # ℓ_A({A_1, A_2}) is f_t''(p)/2, for every slice count
+    t, lam, mu = 0.7, 0.8, 0.3
+    for dt in (0.1, 0.01):
+        m = OracleModel(f"(A:{t},B:0.4);", [], dt)
+        c = m.ccds([TWO_PARALOGS])
+        r = DLWGD([math.log(lam)] * 3, [math.log(mu)] * 3, [], 0.9, [p, 0.2])
+        ell = m.ell(c, 0, r)
+        R = {d["name"]: d for d in m.nodes()}["A"]["k"] + 1
+        A = ell[:3 * R].reshape(3, R).T
+        f = lambda s: bdp_pgf(lam, mu, t, s)
+        assert abs(A[-1, 0] - float(mp.diff(f, p))) <= 1e-12
+        assert abs(A[-1, 2] - float(mp.diff(f, p, 2) / 2)) <= 1e-12
+
+
+def test_dual_gradient_with_leaf_sampling():
+    m = kat_model(10000)
+    c = m.ccds(example1_texts()[:3])
+    rng = np.random.default_rng(21)
+    lam = list(rng.normal(-0.5, 0.3, 17))
+    mu = list(rng.normal(-0.8, 0.3, 17))
+    p = list(rng.uniform(0.05, 0.4, 9))
+    r = DLWGD(lam, mu, [0.3, 0.15], 0.85, p)
+    v, g = m.gradient(c, r)
+    x = r.pack()[1]
+    h = 1e-6
+    for i in list(range(0, 34, 4)) + [34, 35, 36]:
+        xp, xm = x.copy(), x.copy()
+        xp[i] += h
+        xm[i] -= h
+        fp = m.logpdf(c, DLWGD(list(xp[:17]), list(xp[17:34]), list(xp[34:36]), xp[36], p))
+        fm = m.logpdf(c, DLWGD(list(xm[:17]), list(xm[17:34]), list(xm[34:36]), xm[36], p))
+        assert abs((fp - fm) / (2 * h) - g[i]) <= 1e-5 * max(1.0, abs(g[i])), i
+
+
+# ------------------------------------------------------------------ conditions by the extinction ODE
+def _extinction_by_ode(m, r, eta_terms=200):
+    """P(no gene is observed below f, below g, and below both, for the
+    children f, g of the root), from the Kolmogorov backward equation of the
+    linear BDP, du/dτ = μ − (λ+μ)u + λu², integrated numerically along every
+    branch from u = ε at its bottom (a leaf loses each gene with probability
+    p, a WGD duplicates each lineage with probability q, a speciation needs
+    extinction in both subtrees), and an explicit sum over the geometric
+    number of root lineages.  Shares nothing with the closed-form slice
+    recursion (src/bdputil.jl:3-11) or with geompgf (src/bdputil.jl:67)."""
+    nodes = {d["id"]: d for d in m.nodes()}
+    kids = {i: [] for i in nodes}
+    for d in nodes.values():
+        if d["parent"]:
+            kids[d["parent"]].append(d["id"])
+    for v in kids.values():
+        v.sort()
+
+    def rates_of(i):
+        j = i
+        while nodes[j]["wgd"]:
+            j = kids[j][0]
+        return mp.e ** mp.mpf(r.lam[j - 1]), mp.e ** mp.mpf(r.mu[j - 1])
+
+    def top(i):
+        d = nodes[i]
+        if d["leaf"]:
+            u0 = mp.mpf(r.p[i - 1] if len(r.p) else 0.0)
+        elif d["wgd"]:
+            q, e = mp.mpf(r.q[d["wgdid"] - 1]), top(kids[i][0])
+            u0 = q * e * e + (1 - q) * e
+        else:
+            u0 = top(kids[i][0]) * top(kids[i][1])
+        lam, mu = rates_of(i)
+        sol = mp.odefun(lambda tau, u: mu - (lam + mu) * u + lam * u * u, 0, u0)
+        return sol(mp.mpf(d["k"] * d["delta"]))
+
+    with mp.workdps(25):
+        root = next(d["id"] for d in nodes.values() if d["root"])
+        f, g = kids[root]
+        ef, eg = top(f), top(g)
+        eta = mp.mpf(r.eta)
+        geo = [eta * (1 - eta) ** (n - 1) for n in range(1, eta_terms)]
+        ext_f = sum(w * ef ** n for n, w in enumerate(geo, 1))
+        ext_g = sum(w * eg ** n for n, w in enumerate(geo, 1))
+        ext_both = sum(w * (ef * eg) ** n for n, w in enumerate(geo, 1))
+        return float(ext_f), float(ext_g), float(ext_both)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_conditions_match_extinction_ode(seed):
+    """NonExtinctCondition (src/condition.jl:15-19) is log P(some gene is
+    observed), RootCondition (src/condition.jl:21-29) log P(genes observed
+    below both children of the root): both checked against the extinction
+    ODE on a tree with a WGD and leaf sampling p ≠ 0."""
+    rng = np.random.default_rng(100 + seed)
+    tree, ins = "((A:0.6,B:0.4):0.5,C:0.9);", [("wgd_1", "A", "A")]
+    m = OracleModel(tree, ins, 0.05, condition="nonextinct")
+    n = m.n_nonwgd
+    lam = list(np.log(rng.uniform(0.15, 0.9, n)))
+    mu = list(np.log(rng.uniform(0.15, 0.9, n)))
+    r = DLWGD(lam, mu, [float(rng.uniform(0.1, 0.7))], float(rng.uniform(0.5, 0.95)),
+              list(rng.uniform(0.05, 0.5, 3)))
+    ale = TWO_PARALOGS.replace("B_1", "C_1")
+    ext_f, ext_g, ext_both = _extinction_by_ode(m, r)
+    got = m.condition_value(m.ccds([ale]), r)
+    assert abs(got - math.log(1.0 - ext_both)) <= 1e-12 * max(1.0, abs(got))
+    mr = OracleModel(tree, ins, 0.05, condition="root")
+    got = mr.condition_value(mr.ccds([ale]), r)
+    assert abs(got - math.log(1.0 - ext_f - ext_g + ext_both)) <= 1e-12 * max(1.0, abs(got))
+
+
+# ------------------------------------------------------------------ MUL trees: leafℙ = 1/m
+TWO_GENES = ("#constructor_string\n(A_1:1,A_2:1);\n#observations\n10\n#Bip_counts\n#Bip_bls\n1\t1\n2\t1\n"
+             "#Dip_counts\n#last_leafset_id\n2\n#leaf-id\nA_1\t1\nA_2\t2\n#set-id\n1\t:\t1\n2\t:\t2\n#END\n")
+
+
+def mul_two_copies_by_hand(t1, t2, lam, mu, eta, p1, p2):
+    """log ℓ of the two-gene family {A_1, A_2} on the MUL tree (A:t1,A:t2),
+    expanded by hand.  Both copies of A are leaves with leafℙ = 1/2
+    (src/model.jl:110-111) and share A's clade id, so each gene, and the pair,
+    is compatible with both.  Along a leaf branch with pgf f and ε = p at its
+    bottom, a one-gene clade reaches f'(p)·leafℙ at the top and the pair
+    f''(p)/2·leafℙ² (the slice recursion composes pgf derivatives); the root
+    term is src/core.jl:147."""
+    c = mp.mpf(1) / 2
+    f = lambda s: bdp_pgf(lam, mu, t1, s)
+    g = lambda s: bdp_pgf(lam, mu, t2, s)
+    ef, eg = f(p1), g(p2)
+    Lf1, Lg1 = c * mp.diff(f, p1), c * mp.diff(g, p2)
+    Lf2, Lg2 = c * c * mp.diff(f, p1, 2) / 2, c * c * mp.diff(g, p2, 2) / 2
+    eta = mp.mpf(eta)
+    e = ef * eg
+    xi = 1 - (1 - eta) * e
+    A = (1 - eta) * xi / eta
+    B = eta * (1 - e) / xi ** 2
+    Lr1 = B * (Lf1 * eg + Lg1 * ef)  # each single-gene clade at the root
+    Lr = A * Lr1 * Lr1 + B * (Lf1 * Lg1 + Lg1 * Lf1 + Lf2 * eg + Lg2 * ef)
+    return float(mp.log(Lr))
+
+
+@pytest.mark.parametrize("p", [(0.0, 0.0), (0.2, 0.35)])
+@pytest.mark.parametrize("rates", [(0.3, 0.2), (0.5, 0.5)])
+def test_mul_tree_leafp_hand_expanded(rates, p):
+    lam, mu = rates
+    for dt in (0.05, 0.01):
+        m = OracleModel("(A:0.7,A:0.4);", [], dt)
+        assert [d["leafP"] for d in m.nodes()] == [0.5, 0.5, 0.0]
+        c = m.ccds([TWO_GENES])
+        r = DLWGD([math.log(lam)] * 3, [math.log(mu)] * 3, [], 0.8, list(p))
+        got = m.logpdf(c, r, per_family=True)[0]
+        want = mul_two_copies_by_hand(0.7, 0.4, lam, mu, 0.8, *p)
+        assert abs(got - want) <= 1e-12 * abs(want), (got, want)

@@ -43,8 +43,9 @@ class NowhereExtinctCondition(SamplingCondition):
     """Non-extinction at every leaf (src/condition.jl:5-9,31-36): inclusion–
     exclusion over the 2^L evaluations of the tree's pgf at all-binary
     arguments (src/bdputil.jl:107-130), so for trees with few leaves.  The
-    device evaluates the unconditioned values; the condition is applied on
-    the host (no gradient)."""
+    device evaluates the unconditioned values; the condition and its
+    gradient (complex-step differentiation of the same pgf sum) are applied
+    on the host."""
     code = _lib.COND_NONE
     host = True
 
@@ -330,17 +331,23 @@ class WhaleModel:
         vector (DLWGD: [λ; μ; q; η] on log scale; ConstantDLWGD: [λ, μ, q; η])."""
         lam, mu, q, eta, p, ri = self.node_params()
         r = self.rates
+        # A node whose rates do not enter the likelihood (the root: one row,
+        # no slices) has an exact zero derivative; its rate may be NaN
+        # (test/runtests.jl:44-49), so zero terms are skipped rather than
+        # multiplied, which keeps NaN·0 out of the sums as ForwardDiff does.
         if isinstance(r, ConstantDLWGD):
-            gl = float(np.nansum(d_lam))
-            gm = float(np.nansum(d_mu))
+            gl = float(np.sum(d_lam[d_lam != 0]))
+            gm = float(np.sum(d_mu[d_mu != 0]))
             return np.concatenate([[gl, gm], d_q[:len(r.q)], [d_eta]])
         nl = len(r.λ)
         gl = np.zeros(nl)
         gm = np.zeros(nl)
         for e in range(self.s.n):
             if ri[e] >= 0:
-                gl[ri[e]] += lam[e] * d_lam[e]
-                gm[ri[e]] += mu[e] * d_mu[e]
+                if d_lam[e] != 0:
+                    gl[ri[e]] += lam[e] * d_lam[e]
+                if d_mu[e] != 0:
+                    gm[ri[e]] += mu[e] * d_mu[e]
         return np.concatenate([gl, gm, d_q[:len(r.q)], [d_eta]])
 
     def __repr__(self):
