@@ -75,16 +75,20 @@ def copy_probe(nbytes=1 << 28):
     log(f"copy probe: {nbytes} bytes device-to-device")
 
 
-def sampler(w, _lib, synth, args):
-    """Sampler-style loop (config5): device-resident CCDs, parameter points
-    drawn as log-normal perturbations of the truth, evaluated back to back."""
-    wl = synth.workload("config5" if args.workload == "config4" else args.workload, n_families=args.families)
+def sampler(w, synth, workload="config5", n_points=256, n_families=None):
+    """Sampler-style loop (config5: 20 taxa + 5 WGDs, 5,000 families):
+    device-resident CCDs, parameter points drawn as log-normal perturbations
+    of the truth, each evaluated (logL + gradient, FP32-scaled, batched)
+    back to back as HMC leapfrog steps would.  Latency is host call to host
+    result, parameters in and gradient out included."""
+    t0 = time.time()
+    wl = synth.workload(workload, n_families=n_families)
     model = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt)
     ccds = w.ccds_from_texts(wl.texts, model)
-    batch = w.device_batch(model, ccds)
-    n_fam = batch.n
-    points = synth.perturbed(wl.rates, 256)
+    n_fam = w.device_batch(model, ccds).n
+    points = synth.perturbed(wl.rates, n_points)
     models = [model(r) for r in points]
+    log(f"sampler workload {workload}: {n_fam} families ready in {time.time() - t0:.1f}s")
     for m in models[:3]:
         w.gradient(m, ccds, mode="fp32")
     lat = []
@@ -95,11 +99,79 @@ def sampler(w, _lib, synth, args):
         lat.append(time.perf_counter() - t)
     total = time.perf_counter() - t0
     lat = np.array(lat) * 1e3
-    out = {"metric": "sampler loop: logL+grad per parameter point (FP32-scaled, batched)",
-           "workload": f"{wl.name}: {n_fam} families, {len(points)} points",
+    out = {"workload": f"{wl.name}: {n_fam} families, {len(points)} parameter points back to back, "
+                       "logL+gradient FP32-scaled, device-resident CCDs",
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "evals_per_s": len(points) / total, "families_per_s": n_fam * len(points) / total}
-    print(json.dumps(out), flush=True)
+    log(f"sampler: p50 {out['p50_ms']:.3f} ms, p99 {out['p99_ms']:.3f} ms, {out['evals_per_s']:.1f} evals/s")
+    return out
+
+
+def cpu_threads():
+    """Host threads the CPU baseline uses: every hardware thread of this
+    process's CPU set, at most 16 (one GPU's share of a shared 8-GPU host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(wl, n_fam, threads, seconds, k=None):
+    """The oracle (C++ FP64 restatement of the reference: FP64 logL plus a
+    ForwardDiff-style gradient in chunks of 12 duals) on every k-th family,
+    k chosen so that the gradient pass takes about `seconds` unless given;
+    families/s."""
+    from oracle.oracle import OracleModel, DLWGD as ODLWGD
+    om = OracleModel(wl.newick, [], wl.dt)
+    r = wl.rates
+    orr = ODLWGD(list(r.λ), list(r.μ), list(r.q), r.η)
+    fixed = k is not None
+    k = k if fixed else max(1, n_fam // 64)
+    while True:
+        idx = list(range(0, n_fam, k))
+        oc = om.ccds([wl.texts[i] for i in idx])
+        t = time.perf_counter()
+        om.gradient(oc, orr, nthreads=threads)
+        dt = time.perf_counter() - t
+        if fixed or dt * 1.5 > seconds or k == 1:
+            break
+        k = max(1, int(k * dt / seconds))
+    return dict(value=len(idx) / dt, unit="families/s", cores=threads, kind="port", every=k,
+                sample=f"logL+ForwardDiff-style gradient (chunks of 12 duals) of {len(idx)} of the {n_fam} "
+                       f"families (every {k}th), oracle C++ FP64 restatement of the reference, {dt:.1f} s on "
+                       f"{threads} threads")
+
+
+def cpu_calibration():
+    """The reference's own benchmark workload (test/runtests.jl:21-42: the 12
+    example-1 families on the 9-taxon tree with 2 WGDs, Δt = 0.05,
+    RootCondition) timed on the oracle at 2 threads, to be read beside the
+    reference's 220.969 µs (logpdf) and 2.723 ms (ForwardDiff gradient) on
+    its authors' 2-thread machine."""
+    import glob
+    from oracle.oracle import OracleModel, DLWGD as ODLWGD
+    extree = ("((MPOL:4.752,PPAT:4.752):0.292,(SMOE:4.457,(((OSAT:1.555,(ATHA:0.5548,CPAP:0.5548):1.0002):0"
+              ".738,ATRI:2.293):1.225,(GBIL:3.178,PABI:3.178):0.34):0.939):0.587);")
+    files = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "example-1", "*.ale")))
+    om = OracleModel(extree, [("wgd_1", "ATHA", "ATHA"), ("wgd_2", "ATHA", "ATRI")], 0.05)
+    oc = om.ccds([open(f).read() for f in files])
+    r = ODLWGD([1.0] * 17, [1.0] * 17, [0.2, 0.1], 0.9)
+    v = om.logpdf(oc, r, nthreads=2)
+
+    def median_time(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+    t_l = median_time(lambda: om.logpdf(oc, r, nthreads=2), 200)
+    t_g = median_time(lambda: om.gradient(oc, r, nthreads=2), 50)
+    return {"workload": "KAT-2: 12 example-1 families, 9-taxon tree + 2 WGDs, dt=0.05 (test/runtests.jl:21-42)",
+            "threads": 2, "logL": v, "logpdf_us": 1e6 * t_l, "gradient_ms": 1e3 * t_g,
+            "reference_logpdf_us": 220.969, "reference_gradient_ms": 2.723,
+            "note": "reference figures from its test file, on its authors' unnamed 2-thread machine"}
 
 
 def main():
@@ -110,10 +182,10 @@ def main():
     ap.add_argument("--workload", default="config4")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-per-family", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--families", type=int, default=None)
-    ap.add_argument("--sampler", action="store_true",
-                    help="config5 sampler loop: 256 perturbed parameter points, logL+grad each; p50/p99 latency")
+    ap.add_argument("--no-sampler", action="store_true", help="skip the config5 sampler-loop leg")
+    ap.add_argument("--sampler-points", type=int, default=256)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device-to-device copy of 256 MiB (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -127,9 +199,6 @@ def main():
     from whale_jl_amd import _lib, synth
     from whale_jl_amd._lib import check, dblp, ptr
     L = _lib.lib()
-    if args.sampler:
-        return sampler(w, _lib, synth, args)
-
     t0 = time.time()
     wl = synth.workload(args.workload, n_families=args.families)
     log(f"workload {args.workload}: {len(wl.texts)} families generated in {time.time() - t0:.1f}s")
@@ -195,14 +264,16 @@ def main():
     # per-kernel breakdown: the same steps launched eagerly with events between the kernels
     L.whale_ctx_set_timing(ctx, 1)
     ms5 = np.zeros(5)
-    kern = np.zeros(5)
+    runs = []
     nk = max(3, args.steps // 2)
     for _ in range(nk):
         step(_lib.FP32_SCALED)
         L.whale_last_timings(ctx, ptr(ms5, dblp))
-        kern += ms5
+        runs.append(ms5.copy())
     L.whale_ctx_set_timing(ctx, 0)
-    kern /= nk  # [tables, forward, backward, tail, total]
+    kern = np.median(np.array(runs), axis=0)  # [tables, forward, backward, tail, total]
+    log("eager per-phase ms (tables fwd bwd tail), each evaluation: " +
+        "; ".join(" ".join(f"{x:.3f}" for x in r[:4]) for r in runs))
     log(f"fp32-scaled batched: {ms_step:.3f} ms/step (wall {1e3 * wall / args.steps:.3f}), {value:.4g} families/s; "
         f"kernels ms: tables {kern[0]:.3f} fwd {kern[1]:.3f} bwd {kern[2]:.3f} tail {kern[3]:.3f}")
 
@@ -230,36 +301,28 @@ def main():
     fwd_gbs = b_fwd / (kern[1] / 1e3) / 1e9
     bwd_gbs = (b_grad - b_fwd) / (kern[2] / 1e3) / 1e9
     if kern[2] >= kern[1]:
-        dom, ach, traffic_bytes = "k_backward", bwd_gbs, b_grad - b_fwd
+        dom, ach, traffic_bytes, dom_ms = "k_backward", bwd_gbs, b_grad - b_fwd, kern[2]
     else:
-        dom, ach, traffic_bytes = "k_forward", fwd_gbs, b_fwd
+        dom, ach, traffic_bytes, dom_ms = "k_forward", fwd_gbs, b_fwd, kern[1]
     log(f"roofline: fwd {fwd_gbs:.1f} GB/s ({b_fwd / 1e9:.3f} GB), bwd {bwd_gbs:.1f} GB/s "
         f"({(b_grad - b_fwd) / 1e9:.3f} GB), whole step {b_grad / (ms_step / 1e3) / 1e9:.1f} GB/s")
     hbm = measured_traffic(dom) if args.workload == "config4" and args.families is None else None
 
+    # ---- the sampler-style loop (config5), its own workload
+    samp = None
+    if not args.no_sampler:
+        samp = sampler(w, synth, n_points=args.sampler_points)
+
     # ---- CPU baseline: the oracle (C++ restatement) on a bounded sample
-    cpu = None
+    cpu = cpu2 = calib = None
     if not args.no_cpu:
-        from oracle.oracle import OracleModel, DLWGD as ODLWGD
-        ncpu = min(16, os.cpu_count() or 1)
-        om = OracleModel(wl.newick, [], wl.dt)
-        r = wl.rates
-        orr = ODLWGD(list(r.λ), list(r.μ), list(r.q), r.η)
-        # sample every k-th family, growing until the gradient pass takes about cpu_seconds
-        k = max(1, n_fam // 8)
-        while True:
-            idx = list(range(0, n_fam, k))
-            oc = om.ccds([wl.texts[i] for i in idx])
-            t = time.perf_counter()
-            om.gradient(oc, orr, nthreads=ncpu)
-            dt = time.perf_counter() - t
-            if dt * 2.5 > args.cpu_seconds or k == 1:
-                break
-            k = max(1, int(k * dt / args.cpu_seconds * 2) or 1)
-        cpu = dict(value=len(idx) / dt, unit="families/s", cores=ncpu, kind="port",
-                   sample=f"logL+ForwardDiff-style gradient (chunks of 12 duals) of {len(idx)} of the {n_fam} "
-                          f"families (every {k}th), oracle C++ FP64 restatement, {dt:.1f} s on {ncpu} threads")
+        cpu = cpu_baseline(wl, n_fam, cpu_threads(), args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
+        # the same families' share of work at 2 threads: a sample as many times smaller
+        cpu2 = cpu_baseline(wl, n_fam, 2, args.cpu_seconds, k=cpu["every"] * max(1, cpu["cores"] // 2))
+        log(f"cpu baseline at 2 threads: {cpu2}")
+        calib = cpu_calibration()
+        log(f"cpu calibration: {calib}")
 
     out = {
         "metric": "families/s for logL+grad: FP32-scaled batched vs FP64 plain, rel err ≤1e-6",
@@ -279,14 +342,20 @@ def main():
                    "families": n_fam, "cells": st["cells"], "split_terms": st["split_terms"]},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": hbm, "kernel": dom,
-                     "algorithmic_bytes": traffic_bytes},
+                     "algorithmic_bytes": traffic_bytes, "duration_ms": dom_ms,
+                     "duration": "HIP events on the library's stream around the kernel's phase (all size-class "
+                                 "launches and the FP64 families, several streams) of eager evaluations, median"},
         "cpu_baseline": cpu,
+        "cpu_baseline_2_threads": cpu2,
+        "cpu_calibration_kat2": calib,
         "accuracy": {"fp32_vs_fp64_family_max_rel": rel_fam, "fp32_vs_fp64_total_rel": rel_tot,
                      "fp32_vs_fp64_grad_rel": gerr, "fp64_plain_underflowed_families": n_underflow,
                      "fp32_families_recomputed_in_fp64": n_fallback},
         "fp64_plain_per_family_families_per_s": per_fam,
         "speedup_vs_fp64_per_family": (value / per_fam) if per_fam else None,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
+        "speedup_vs_cpu_2_threads": (value / cpu2["value"]) if cpu2 else None,
+        "sampler": samp,
         "fp64_plain_batched_ms": fp64_batched_ms,
         "kernel_ms": {"tables": kern[0], "forward": kern[1], "backward": kern[2], "tail": kern[3]},
     }

@@ -160,6 +160,18 @@ int whale_batch_fallbacks(const whale_batch* b, int32_t* n);
 #define WHALE_SIZE_CLASSES 6
 int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* out);
 
+/* Families whose largest stage does not fit one compute unit's LDS in the
+ * FP64-scaled form (the reference evaluates families of any size, src/core.jl:
+ * 83-158; example/example-4/cytp450.ale holds 4,917 clades): they run one per
+ * workgroup with their stage buffers in a global workspace.  *n receives how
+ * many families of the batch take that path.  No reference counterpart. */
+int whale_batch_hbm_families(const whale_batch* b, int32_t* n);
+/* Lowers the LDS a family's stages may use before it takes that path (bytes;
+ * 0 restores the device's limit) and re-plans the batch: with a small limit
+ * every family runs with its stage buffers in global memory.  No reference
+ * counterpart (testing and tuning). */
+int whale_batch_set_lds_limit(whale_batch* b, int64_t bytes);
+
 /* logpdf of every family (unconditioned, src/core.jl:31-43) into
  * out_family[n_families] (may be NULL) and the conditioned batch value
  * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */

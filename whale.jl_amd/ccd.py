@@ -176,6 +176,17 @@ class DeviceBatch:
         check(_lib.lib().whale_batch_fallbacks(self.h, C.byref(n)), "whale_batch_fallbacks")
         return n.value
 
+    def hbm_families(self) -> int:
+        """Families whose stages run in global memory (they do not fit one compute unit's LDS)."""
+        n = C.c_int32()
+        check(_lib.lib().whale_batch_hbm_families(self.h, C.byref(n)), "whale_batch_hbm_families")
+        return n.value
+
+    def set_lds_limit(self, nbytes: int):
+        """Re-plans with at most `nbytes` of LDS per family's stages (0: the device's
+        limit); families above it run with their stage buffers in global memory."""
+        check(_lib.lib().whale_batch_set_lds_limit(self.h, int(nbytes)), "whale_batch_set_lds_limit")
+
     def launches(self, mode: str = "fp32") -> list:
         """Launch shape of each batched kernel: forward and reverse sweep per size class."""
         ncls = _lib.SIZE_CLASSES

@@ -62,10 +62,14 @@ hipError_t upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
 struct whale_batch;
 namespace {
 // family tickets: one per class and direction, two for the FP64 pass after an
-// FP32 pass, two for the families routed to FP64 from the start
-constexpr int kTickets = 16;
+// FP32 pass, two for the families routed to FP64 from the start, two for the
+// families whose stages do not fit LDS
+constexpr int kTickets = 18;
+constexpr int kTicketHbm = 16;
 int assign_classes(whale_batch* b);
 int plan_batch_launches(whale_batch* b);
+int ensure_gws(whale_batch* b, bool* moved);
+void drop_graphs(whale_batch* b);
 
 // LDS one wave needs for family d: its largest non-root stage (items and
 // records bounded separately) and its root stage, root records staged or not
@@ -132,11 +136,24 @@ struct whale_batch {
   // their first 8 bytes per element and, for FP32 runs, the FP64 recomputation
   // of families that left the FP32 range after the first align8(4 * n) bytes.
   DBuf ell, coef, E, ibar, fam_ll, acc, ticket, flag, fb;
+  // stage buffers of the launches that run them in global memory
+  // (LaunchCfg::gws): the launches on the context's stream use the first
+  // gws_main bytes, the FP64 evaluation of routed families (on its own
+  // stream, concurrently) the rest
+  DBuf gws;
+  size_t gws_main = 0;
   uint64_t n_rec = 0;
   uint32_t n_fam = 0;
   // size classes by the LDS one family needs: families order[cls_lo[c] .. cls_hi[c])
   static constexpr int kClasses = 6;
   uint32_t cls_lo[kClasses] = {}, cls_hi[kClasses] = {};
+  // Families whose largest stage, in the FP64-scaled form, does not fit the
+  // LDS of one compute unit (or `lds_limit`, when set): order[hbm_lo ..
+  // hbm_hi), first in `order`; they run one per workgroup with their stage
+  // buffers in the global workspace `gws`.
+  uint32_t hbm_lo = 0, hbm_hi = 0;
+  size_t lds_limit = 0;  // 0: the device's LDS per workgroup
+  LaunchCfg cfg_hbm[3][2];
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   int last_mode = -1;
@@ -194,6 +211,7 @@ struct whale_batch {
     W.ticket = ticket.as<uint32_t>();
     W.flag = flag.as<uint32_t>();
     W.fb = fb.as<uint32_t>();
+    W.gws = gws.as<unsigned char>();
     return W;
   }
   // the FP64 region used for families recomputed after an FP32 pass
@@ -573,6 +591,10 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   whale_ctx* c = m->ctx;
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->stream;
+  {
+    const int rc = ensure_gws(b.get(), nullptr);
+    if (rc != WHALE_OK) return rc;
+  }
   HIPCHK(upload(b->fam, P.fam, st));
   HIPCHK(upload(b->stage, P.stage, st));
   HIPCHK(upload(b->seg, P.seg, st));
@@ -649,6 +671,25 @@ int whale_batch_launches(const whale_batch* b, int32_t mode, int64_t* o) {
   return WHALE_OK;
 }
 
+int whale_batch_hbm_families(const whale_batch* b, int32_t* n) {
+  if (!b || !n) return fail(WHALE_EINVAL, "NULL argument");
+  *n = (int32_t)(b->hbm_hi - b->hbm_lo);
+  return WHALE_OK;
+}
+
+int whale_batch_set_lds_limit(whale_batch* b, int64_t bytes) {
+  if (!b) return fail(WHALE_EINVAL, "NULL argument");
+  if (bytes < 0) return fail(WHALE_EINVAL, "negative LDS limit");
+  HIPCHK(hipSetDevice(b->device));
+  b->lds_limit = (size_t)bytes;
+  int rc = assign_classes(b);
+  if (rc != WHALE_OK) return rc;
+  if ((rc = ensure_gws(b, nullptr)) != WHALE_OK) return rc;
+  HIPCHK(hipMemcpy(b->order.p, b->order_host.data(), sizeof(uint32_t) * b->n_fam, hipMemcpyHostToDevice));
+  drop_graphs(b);
+  return WHALE_OK;
+}
+
 int whale_batch_stats(const whale_batch* b, int64_t* o) {
   if (!b || !o) return fail(WHALE_EINVAL, "NULL argument");
   const PlanHost& P = b->plan;
@@ -718,7 +759,7 @@ bool plan_launch(const whale_model* m, bool fwd, int mode, size_t wave, uint64_t
 // them, with the root records staged in LDS unless reading them from global
 // memory lets more waves share a compute unit
 bool plan_families(const whale_batch* b, bool fwd, int mode, const uint32_t* fams, size_t nf, uint64_t grid_fams,
-                   bool multi, LaunchCfg* out) {
+                   bool multi, LaunchCfg* out, bool force_gws = false) {
   size_t with = 16, without = 16;
   for (size_t j = 0; j < nf; ++j) {
     const FamDev& d = b->plan.fam[fams[j]];
@@ -726,17 +767,54 @@ bool plan_families(const whale_batch* b, bool fwd, int mode, const uint32_t* fam
     without = std::max(without, fam_lds(d, mode, fwd, false));
   }
   LaunchCfg a, c;
-  const bool oa = plan_launch(b->model, fwd, mode, with, grid_fams, multi, &a);
-  const bool oc = plan_launch(b->model, fwd, mode, without, grid_fams, multi, &c);
+  const bool oa = !force_gws && plan_launch(b->model, fwd, mode, with, grid_fams, multi, &a);
+  const bool oc = !force_gws && plan_launch(b->model, fwd, mode, without, grid_fams, multi, &c);
   if (oa && (!oc || a.waves_per_cu >= c.waves_per_cu)) *out = a;
   else if (oc) *out = c;
-  return oa || oc;
+  if (oa || oc) return true;
+  // the stage buffers in global memory: one wave per workgroup, root records
+  // read from global memory, only the node constants (and accumulators) in LDS
+  const whale_model* m = b->model;
+  const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+  LaunchCfg g;
+  g.gws = true;
+  g.fpw = 1;
+  g.tabl = false;
+  g.waves_per_cu = 1;
+  g.L.wave_bytes = (uint32_t)align16(without);
+  g.L.shared_bytes = (uint32_t)(fwd ? fwd_shared_bytes(mode, false, m->h.tab_rows, m->h.n, m->h.n_wgd)
+                                    : bwd_shared_bytes(mode, false, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd));
+  if (g.L.shared_bytes > m->ctx->lds_wg || without > 0xffffffffull) return false;
+  g.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)m->ctx->n_cu, grid_fams));
+  *out = g;
+  return true;
+}
+
+// whether family d runs with its stage buffers in LDS: its FP64-scaled
+// stages (the largest form) fit the LDS limit in both directions
+bool fits_lds(const whale_batch* b, const FamDev& d) {
+  const whale_model* m = b->model;
+  const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+  size_t lim = m->ctx->lds_wg;
+  if (b->lds_limit) lim = std::min(lim, b->lds_limit);
+  const int md = WHALE_FP64_SCALED;
+  const size_t f = fwd_shared_bytes(md, false, m->h.tab_rows, m->h.n, m->h.n_wgd) +
+                   align16(std::max<size_t>(16, fam_lds(d, md, true, false)));
+  const size_t r = bwd_shared_bytes(md, false, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd) +
+                   align16(std::max<size_t>(16, fam_lds(d, md, false, false)));
+  return f <= lim && r <= lim;
 }
 
 // launch shapes of every class, mode and direction, and of the FP64 pass over
 // families the FP32 form could not represent
 int plan_batch_launches(whale_batch* b) {
   const whale_model* m = b->model;
+  for (int mode = 0; mode < 3; ++mode)
+    for (int dir = 0; dir < 2; ++dir)
+      if (b->hbm_hi > b->hbm_lo &&
+          !plan_families(b, dir == 0, mode, b->order_host.data() + b->hbm_lo, b->hbm_hi - b->hbm_lo,
+                         b->hbm_hi - b->hbm_lo, false, &b->cfg_hbm[mode][dir], true))
+        return fail(WHALE_EINVAL, "the node constants do not fit the LDS of one compute unit");
   for (int mode = 0; mode < 3; ++mode)
     for (int dir = 0; dir < 2; ++dir)
       for (int k = 0; k < whale_batch::kClasses; ++k) {
@@ -758,10 +836,14 @@ int plan_batch_launches(whale_batch* b) {
 int assign_classes(whale_batch* b) {
   const PlanHost& P = b->plan;
   const size_t bounds[whale_batch::kClasses - 1] = {10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024, 48 * 1024};
-  std::vector<uint32_t> cls[whale_batch::kClasses], rt;
+  std::vector<uint32_t> cls[whale_batch::kClasses], rt, hbm;
   for (uint32_t f : P.cost_order) {
     if (b->routed[f]) {
       rt.push_back(f);
+      continue;
+    }
+    if (!fits_lds(b, P.fam[f])) {
+      hbm.push_back(f);
       continue;
     }
     const size_t need = std::max(fam_lds(P.fam[f], WHALE_FP32_SCALED, true, true),
@@ -771,6 +853,9 @@ int assign_classes(whale_batch* b) {
     cls[k].push_back(f);
   }
   b->order_host.clear();
+  b->hbm_lo = 0;
+  b->order_host.insert(b->order_host.end(), hbm.begin(), hbm.end());
+  b->hbm_hi = (uint32_t)b->order_host.size();
   for (int k = whale_batch::kClasses - 1; k >= 0; --k) {
     b->cls_lo[k] = (uint32_t)b->order_host.size();
     b->order_host.insert(b->order_host.end(), cls[k].begin(), cls[k].end());
@@ -787,6 +872,40 @@ int assign_classes(whale_batch* b) {
                          rt.size(), true, &b->cfg_route[mode][dir]))
         return fail(WHALE_EINVAL, "a family's largest stage does not fit the LDS of one compute unit");
   return WHALE_OK;
+}
+
+// The global workspace for the stage buffers of every launch planned with
+// them in global memory (the class of families that do not fit LDS, the FP64
+// passes over flagged or routed families, and per-family launches of the
+// former).  Grows only; captured evaluations must be dropped when it moves.
+int ensure_gws(whale_batch* b, bool* moved) {
+  size_t main = 0, route = 0;
+  for (int mode = 0; mode < 3; ++mode)
+    for (int dir = 0; dir < 2; ++dir) {
+      if (b->hbm_hi > b->hbm_lo) main = std::max(main, b->cfg_hbm[mode][dir].gws_bytes());
+      if (b->route_lo < b->n_fam) route = std::max(route, b->cfg_route[mode][dir].gws_bytes());
+    }
+  for (int dir = 0; dir < 2; ++dir) main = std::max(main, b->cfg_fb[dir].gws_bytes());
+  for (uint32_t k = b->hbm_lo; k < b->hbm_hi; ++k)
+    for (int dir = 0; dir < 2; ++dir)
+      main = std::max(main, align16(fam_lds(b->plan.fam[b->order_host[k]], WHALE_FP64_SCALED, dir == 0, false)));
+  main = (main + 255) & ~(size_t)255;
+  const size_t need = main + route;
+  if (moved) *moved = main != b->gws_main || need > b->gws.n;
+  b->gws_main = main;
+  if (need == 0 || need <= b->gws.n) return WHALE_OK;
+  HIPCHK(b->gws.alloc(need));
+  return WHALE_OK;
+}
+
+void drop_graphs(whale_batch* b) {
+  for (auto& a : b->graph)
+    for (auto& b2 : a)
+      for (auto& g : b2)
+        if (g) {
+          (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
 }
 
 // All device work of one evaluation on the context's stream (capturable: no
@@ -829,6 +948,13 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     HIPCHK(hipEventRecord(c->fork[3 * dir], st));
     for (int l = 1; l < kLanes; ++l)
       if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], c->fork[3 * dir], 0));
+    if (b->hbm_hi > b->hbm_lo) {  // families with their stage buffers in global memory
+      WorkRaw Wh = W;
+      Wh.ticket = W.ticket + kTicketHbm + dir;
+      const LaunchCfg& C = b->cfg_hbm[mode][dir];
+      if (dir == 0) HIPCHK(launch_forward(mode, store_all, C, st, B, M, Wh, -1, b->hbm_lo, b->hbm_hi));
+      else HIPCHK(launch_backward(mode, C, st, B, M, Wh, -1, b->hbm_lo, b->hbm_hi));
+    }
     for (int j = 0; j < whale_batch::kClasses; ++j) {
       const int k = kOrder[dir][j];
       if (b->cls_hi[k] == b->cls_lo[k]) continue;
@@ -862,6 +988,12 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     HIPCHK(hipEventRecord(c->fork[0], st));
     for (int l = 1; l < kLanes; ++l)
       if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], c->fork[0], 0));
+    const bool hbm = b->hbm_hi > b->hbm_lo;
+    WorkRaw Wh = W;
+    if (hbm) {
+      Wh.ticket = W.ticket + kTicketHbm;
+      HIPCHK(launch_forward(mode, store_all, b->cfg_hbm[mode][0], st, B, M, Wh, -1, b->hbm_lo, b->hbm_hi));
+    }
     for (int j = 0; j < whale_batch::kClasses; ++j) {
       const int k = kOrder[j];
       if (b->cls_hi[k] == b->cls_lo[k]) continue;
@@ -869,6 +1001,10 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
       WorkRaw Wk = W;
       Wk.ticket = W.ticket + k;
       HIPCHK(launch_forward(mode, store_all, b->cfg[mode][0][k], sk, B, M, Wk, -1, b->cls_lo[k], b->cls_hi[k]));
+    }
+    if (hbm) {
+      Wh.ticket = W.ticket + kTicketHbm + 1;
+      HIPCHK(launch_backward(mode, b->cfg_hbm[mode][1], st, B, M, Wh, -1, b->hbm_lo, b->hbm_hi));
     }
     for (int j = 0; j < whale_batch::kClasses; ++j) {
       const int k = kOrder[j];
@@ -902,6 +1038,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     HIPCHK(hipStreamWaitEvent(c->side2, c->fork[6], 0));
     const int rmode = fp32 ? WHALE_FP64_SCALED : mode;
     WorkRaw Wr = fp32 ? b->work_fallback() : W;
+    if (Wr.gws) Wr.gws += b->gws_main;  // its own stage-buffer region: it runs beside the other launches
     Wr.ticket = W.ticket + 2 * whale_batch::kClasses + 2;
     HIPCHK(launch_forward(rmode, store_all, b->cfg_route[mode][0], c->side2, B, M, Wr, -1, b->route_lo, nf));
     if (grad) {
@@ -969,16 +1106,11 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
   if (!b->flagged_once.empty()) {
     for (uint32_t f : b->flagged_once) b->routed[f] = 1;
     b->flagged_once.clear();
-    const int rc = assign_classes(b);
+    int rc = assign_classes(b);
     if (rc != WHALE_OK) return rc;
+    if ((rc = ensure_gws(b, nullptr)) != WHALE_OK) return rc;
     HIPCHK(hipMemcpy(b->order.p, b->order_host.data(), sizeof(uint32_t) * b->n_fam, hipMemcpyHostToDevice));
-    for (auto& a : b->graph)
-      for (auto& b2 : a)
-        for (auto& g : b2)
-          if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-          }
+    drop_graphs(b);
   }
   if (launch == WHALE_LAUNCH_BATCHED && !c->timing) {
     // replay the captured evaluation (captured on first use)

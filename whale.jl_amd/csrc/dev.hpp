@@ -52,6 +52,7 @@ struct WorkRaw {
   uint32_t* ticket;
   uint32_t* flag;     // per family: 1 if the FP32 form left its range (recomputed in FP64)
   uint32_t* fb;       // fb[0] = number of such families, fb[1..] = their indices
+  unsigned char* gws; // stage buffers of launches whose families do not fit LDS (LaunchCfg::gws)
 };
 
 // gradient accumulator layout (doubles)
@@ -105,14 +106,19 @@ struct LdsCfg {
   uint32_t shared_bytes;
 };
 
-// a launch: `fpw` families (waves) per workgroup, slice tables in LDS or not
+// a launch: `fpw` families (waves) per workgroup, slice tables in LDS or not.
+// With `gws` the waves' stage buffers (wave_bytes each) live in a global
+// workspace instead of LDS, one wave per workgroup: the path of families whose
+// largest stage does not fit one compute unit's LDS.
 struct LaunchCfg {
   int grid = 1;
   int fpw = 1;
   int waves_per_cu = 0;  // resident waves per compute unit the LDS allows
   bool tabl = false;
+  bool gws = false;
   LdsCfg L{};
-  size_t lds() const { return (size_t)L.shared_bytes + (size_t)fpw * L.wave_bytes; }
+  size_t lds() const { return (size_t)L.shared_bytes + (gws ? 0 : (size_t)fpw * L.wave_bytes); }
+  size_t gws_bytes() const { return gws ? (size_t)grid * fpw * L.wave_bytes : 0; }
 };
 
 size_t fwd_shared_bytes(int mode, bool tabl, int tab_rows, int n, int nwgd);

@@ -65,9 +65,10 @@ struct WorkDev {
   uint32_t* ticket;
   uint32_t* flag;
   uint32_t* fb;
+  unsigned char* gws;
   explicit WorkDev(const WorkRaw& w)
       : ell((Real*)w.ell), coef((Real*)w.coef), E(w.E), ibar((Real*)w.ibar), fam_ll(w.fam_ll), acc(w.acc),
-        ticket(w.ticket), flag(w.flag), fb(w.fb) {}
+        ticket(w.ticket), flag(w.flag), fb(w.fb), gws(w.gws) {}
 };
 
 __device__ __forceinline__ double eps_top(const ModelDev& M, int node) {
@@ -317,6 +318,15 @@ __device__ __forceinline__ void wave_sync_mem() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
+// The synchronisation between a wave's passes over its stage buffers: those
+// buffers are LDS, or (Gws) the wave's region of a global workspace for
+// families whose stages do not fit one compute unit's LDS, where every pass
+// must also wait for the previous pass's global stores.
+template <bool Gws>
+__device__ __forceinline__ void wsync() {
+  if (Gws) wave_sync_mem();
+  else wave_sync();
+}
 
 // φ_r, ψ_r of table row `row`: from the workgroup's LDS copy or from global memory
 template <typename Real, bool TabL>
@@ -537,7 +547,7 @@ struct FwdRoot {
 // ---------------------------------------------------------------------------
 constexpr int kMaxFPW = 8;  // families (waves) per workgroup, at most
 
-template <typename Real, bool Scaled, bool StoreAll, bool TabL>
+template <typename Real, bool Scaled, bool StoreAll, bool TabL, bool Gws>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                        const uint32_t* fam_count) {
@@ -556,7 +566,10 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
   }
   NC.fill(M, threadIdx.x, (int)blockDim.x);
   __syncthreads();
-  unsigned char* const wb = smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
+  // the wave's stage buffers: its share of the dynamic LDS, or (Gws) its
+  // region of the global workspace (one wave per workgroup)
+  unsigned char* const wb = Gws ? W.gws + ((size_t)blockIdx.x * (blockDim.x / kWG) + wv) * L.wave_bytes
+                                : smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
   bool first = true;
   for (;;) {
     uint32_t f;
@@ -620,7 +633,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
         sCoef[k] = p;  // the scaled forms rescale it after row 0
         sP[k] = p;
       }
-      wave_sync();
+      wsync<Gws>();
       // Row 0 of a stage without WGD segments: the split terms are summed
       // record-parallel first (into sAcc, which is zero until the row loop),
       // so the item pass below only adds the direct terms.  A clade still
@@ -647,7 +660,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
             if (ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ee); (void)frexp_r(m, &em); emax = max(emax, ee + em); }
             if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
           }
-          wave_sync();
+          wsync<Gws>();
         }
         for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the run sums use DPP
           const uint32_t k = base + lane;
@@ -680,7 +693,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
           t = row_seg_sum(t, own, &last);
           if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
         }
-        wave_sync();
+        wsync<Gws>();
       }
       // row 0: terms from the child branches (whale!/whalewgd!, src/core.jl:95-128)
       auto row0 = [&](const SegInfo& sg, uint32_t i) {
@@ -807,7 +820,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
         const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
         if (i < n) row0(sg, i);
       }
-      wave_sync();
+      wsync<Gws>();
       // scaled forms: coefficients for the exponents now fixed.  A record whose
       // clade is still open keeps p (the open clade sums explicit exponents).
       auto refresh = [&]() {
@@ -834,7 +847,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
       uint32_t n_open = 0;
       if (Scaled) {
         refresh();
-        wave_sync();
+        wsync<Gws>();
         n_open = count_open();
       }
       // the first RC records per lane in registers: operands, owner, coefficient
@@ -892,7 +905,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
           t = row_seg_sum(t, own, &last);
           if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
         }
-        wave_sync();
+        wsync<Gws>();
         // a young clade past 2^kBump is renormalised in place (its only values are
         // this branch's rows); its exponent is raised after the row, as other items
         // of this row still read the clade's previous row at the old scale
@@ -995,7 +1008,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
           const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
           if (i < n) update(it, i);
         }
-        wave_sync();
+        wsync<Gws>();
         if (Scaled && __any(dirty)) {
           for (uint32_t i = lane; i < n; i += kWG) {
             if (sFlag[i] & kShifted) {
@@ -1003,9 +1016,9 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
               sFlag[i] &= (uint8_t)~kShifted;
             }
           }
-          wave_sync();
+          wsync<Gws>();
           refresh();
-          wave_sync();
+          wsync<Gws>();
           n_open = count_open();
           load_recs();
         }
@@ -1063,7 +1076,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
       // cB·(F(γ)ε_g + G(γ)ε_f) (zero for a clade without a value below the
       // root), then the level passes add the split terms of its records.
       for (uint32_t i = lane; i < n; i += kWG) bA[i] = cB * (bF[i] * eg + bB[i] * ef);
-      wave_sync();
+      wsync<Gws>();
       // record k of a level: operands, p, owner
       auto rrec = [&](uint32_t k, uint32_t& ops, Real& p, uint32_t& own) {
         if (inl) { ops = sOps[k]; p = sCoef[k]; own = sOwn[k]; }
@@ -1098,7 +1111,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
             if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
             if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
           }
-          wave_sync();
+          wsync<Gws>();
         }
         for (uint32_t base = k0; base < k1; base += kWG) {  // uniform: the run sums use DPP
           const uint32_t k = base + lane;
@@ -1133,7 +1146,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
           t = row_seg_sum(t, own, &last);
           if (last && own != kNone && t != Real(0)) lds_add(&bA[own], t);
         }
-        wave_sync();
+        wsync<Gws>();
         if (Scaled) {
           for (uint32_t i = a0 + lane; i < a1; i += kWG) {
             const int Eo = sEi[i];
@@ -1144,7 +1157,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
               sEi[i] = Eo - kOpen + e2 + kBirth;
             }
           }
-          wave_sync();
+          wsync<Gws>();
         }
       }
       for (uint32_t i = lane; i < n; i += kWG) {
@@ -1254,7 +1267,7 @@ __device__ __forceinline__ ItemTop item_top(const SegInfo& sg, uint32_t i) {
   return t;
 }
 
-template <typename Real, bool Scaled, bool TabL>
+template <typename Real, bool Scaled, bool TabL, bool Gws>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                         int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                         const uint32_t* fam_count) {
@@ -1283,7 +1296,10 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
   }
   NC.fill(M, threadIdx.x, (int)blockDim.x);
   __syncthreads();
-  unsigned char* const wb = smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
+  // the wave's stage buffers: its share of the dynamic LDS, or (Gws) its
+  // region of the global workspace (one wave per workgroup)
+  unsigned char* const wb = Gws ? W.gws + ((size_t)blockIdx.x * (blockDim.x / kWG) + wv) * L.wave_bytes
+                                : smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
   const Real* coefs = Scaled ? W.coef : nullptr;
   bool first = true;
   for (;;) {
@@ -1343,7 +1359,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           sOwn[k] = B.rec_own[rb + k];
           sCoef[k] = coef_at(rb + k);
         }
-      wave_sync();
+      wsync<Gws>();
       // record k: operands, coefficient, owning clade
       auto rec = [&](uint32_t k, uint32_t& ops, Real& c, uint32_t& own) {
         if (inl) { ops = sOps[k]; c = sCoef[k]; own = sOwn[k]; }
@@ -1353,7 +1369,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
       const bool ok = Lr > Real(0) && isfinite((double)Lr);
       if (ok) {
         if (lane == 0) bA[n - 1] = Real(1) / Lr;
-        wave_sync();
+        wsync<Gws>();
         // d logL / d ℓ of each root clade, larger clades first, scattered into
         // the parts; a level's records (those of its clades) one per lane
         for (int Lv = (int)F.n_lvl - 1; Lv >= 0; --Lv) {
@@ -1369,7 +1385,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
             lds_add(&bA[x], t * bV[y]);
             lds_add(&bA[y], t * bV[x]);
           }
-          wave_sync();
+          wsync<Gws>();
         }
         Real sA = 0, sBc = 0, sF = 0, sG = 0;
         for (uint32_t k = lane; k < nrec; k += kWG) {
@@ -1398,12 +1414,12 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
           acc_add<TabL>(accX + AL.eps + nd.child0, (double)cB * (double)sG);
         }
         // ī of the child tops: child0 slot takes ℓ of child1 tops and vice versa
-        wave_sync();
+        wsync<Gws>();
         for (int slot = 0; slot < 2; ++slot) {
           const Real* other = slot == 0 ? bG : bB;
           const Real eo = slot == 0 ? eg : ef;
           for (uint32_t i = lane; i < n; i += kWG) bV[i] = bA[i] * eo;
-          wave_sync();
+          wsync<Gws>();
           for (uint32_t k = lane; k < nrec; k += kWG) {
             uint32_t ops, own;
             Real c;
@@ -1414,9 +1430,9 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
             lds_add(&bV[x], g * other[y]);
             lds_add(&bV[y], g * other[x]);
           }
-          wave_sync();
+          wsync<Gws>();
           for (uint32_t i = lane; i < n; i += kWG) ibar[sg.ibar + slot * n + i] = cB * bV[i];
-          wave_sync();
+          wsync<Gws>();
         }
       } else {
         for (uint32_t i = lane; i < 2 * n; i += kWG) ibar[sg.ibar + i] = 0;
@@ -1454,7 +1470,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         sOwn[k] = B.rec_own[rb + k];
         sCoef[k] = coef_at(rb + k);
       }
-      wave_sync();
+      wsync<Gws>();
       ItemRow cache[kCacheB];
       ItemTop tops[kCacheB];
       // ℓ of the rows below, loaded two rows ahead into two register sets
@@ -1617,9 +1633,9 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
       };
       // REC(r), then C(r) with A(r-1); A(r-1) reads ℓ(r-2) from `pre`
       auto row_step = [&](const int r, Real (&pre)[kCacheB]) {
-        wave_sync();
+        wsync<Gws>();
         rec_pass(r);
-        wave_sync();
+        wsync<Gws>();
         const bool more = r >= 2;  // uniform
 #pragma unroll
         for (int c = 0; c < kCacheB; ++c) {
@@ -1649,7 +1665,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         row_step(r, preB);
         if (r >= 2) row_step(r - 1, preA);
       }
-      wave_sync();
+      wsync<Gws>();
       if (TabL) {
         // move this stage's d/dφ_r, d/dψ_r into the global FP64 sums (each LDS
         // partial then holds at most one stage's contributions per family)
@@ -1705,14 +1721,14 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         sFo[i] = 0;
         any_wgd |= tp.kind == WHALE_NODE_WGD;
       });
-      wave_sync();
+      wsync<Gws>();
       if (__any(any_wgd)) {
         // WGD retention sums Σ c·ℓ(γ1)ℓ(γ2) over the child's tops, per owner
         each_rec([&](uint32_t ops, uint32_t own, Real c) {
           const Real t = c * bV[ops & 0xffff] * bV[ops >> 16];
           if (t != Real(0)) lds_add(&sFo[own], t);
         });
-        wave_sync();
+        wsync<Gws>();
       }
       // ε̄, q̄ and the direct terms of the child0 slot; then each owner's
       // factor of its split terms in that slot
@@ -1736,7 +1752,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         }
         sFo[i] = fac;
       });
-      wave_sync();
+      wsync<Gws>();
       // split terms of the child0 slot: speciation pairs child0 parts with child1
       // tops of the other part; a WGD pairs both parts on its one child
       each_rec([&](uint32_t ops, uint32_t own, Real c) {
@@ -1746,14 +1762,14 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         lds_add(&bB[a], g * bG[b]);
         lds_add(&bB[b], g * bG[a]);
       });
-      wave_sync();
+      wsync<Gws>();
       each([&](const ItemTop& tp, uint32_t i) {
         if (tp.kind == WHALE_NODE_SPECIATION || tp.kind == WHALE_NODE_WGD) ibar[tp.ib] = bB[i];
         const bool sp = tp.kind == WHALE_NODE_SPECIATION;
         if (sp) bB[i] = bA[i] * NC.etop[tp.c0];
         sFo[i] = sp ? bA[i] : Real(0);
       });
-      wave_sync();
+      wsync<Gws>();
       // child1 slot of speciation nodes
       each_rec([&](uint32_t ops, uint32_t own, Real c) {
         const Real g = sFo[own] * c;
@@ -1762,7 +1778,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
         lds_add(&bB[a], g * bV[b]);
         lds_add(&bB[b], g * bV[a]);
       });
-      wave_sync();
+      wsync<Gws>();
       each([&](const ItemTop& tp, uint32_t i) {
         if (tp.kind == WHALE_NODE_SPECIATION) ibar[tp.ib + tp.nc] = bB[i];
       });
@@ -1928,9 +1944,16 @@ static hipError_t fwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
   if (C.fpw < 1 || C.fpw > kMaxFPW) return hipErrorInvalidValue;
+  if (C.gws) {  // stage buffers in the global workspace: one wave per workgroup, tables from global memory
+    if (C.fpw != 1 || C.tabl || !W.gws) return hipErrorInvalidValue;
+    return go(k_forward<R, S, A, false, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count);
+  }
   if (C.tabl)
-    return go(k_forward<R, S, A, true>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
-  return go(k_forward<R, S, A, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+    return go(k_forward<R, S, A, true, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count);
+  return go(k_forward<R, S, A, false, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+            fam_count);
 }
 
 template <typename R, bool S>
@@ -1940,9 +1963,16 @@ static hipError_t bwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
   if (C.fpw < 1 || C.fpw > kMaxFPW) return hipErrorInvalidValue;
+  if (C.gws) {
+    if (C.fpw != 1 || C.tabl || !W.gws) return hipErrorInvalidValue;
+    return go(k_backward<R, S, false, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count);
+  }
   if (C.tabl)
-    return go(k_backward<R, S, true>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
-  return go(k_backward<R, S, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end, fam_count);
+    return go(k_backward<R, S, true, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count);
+  return go(k_backward<R, S, false, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+            fam_count);
 }
 
 hipError_t launch_forward(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
