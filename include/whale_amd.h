@@ -177,6 +177,24 @@ int whale_batch_set_lds_limit(whale_batch* b, int64_t bytes);
  * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */
 int whale_logpdf(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total);
 
+/* whale_logpdf (batched) that allocates nothing: no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that forbid hipMalloc).  Its workspace is
+ * a fixed device array of WHALE_NOALLOC_SCRATCH_BYTES that every GPU the
+ * library runs on holds until the process ends; a batch that needs more
+ * (ℓ without stored rows: 8 bytes per cell in the FP64 forms, 12 in the FP32
+ * form, plus per-clade exponents and per-family flags) is rejected with
+ * WHALE_EINVAL.  d_family (n_families doubles) and d_total (one double) are
+ * device memory the caller owns; they receive the per-family values and the
+ * conditioned total.  The call is stream-ordered: it returns once the work is
+ * enqueued on the context's stream (whale_ctx_stream), and the results are
+ * there when that stream reaches this point.  The array is shared by every
+ * call on one GPU and keeps the last call's contents (each call clears what
+ * must start at zero): calls that overlap on one GPU, from two contexts or two
+ * threads, race. */
+#define WHALE_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int whale_logpdf_noalloc(whale_batch* b, int32_t mode, double* d_family, double* d_total);
+
 /* logpdf plus its gradient with respect to the natural-scale per-node rates
  * (d_lambda, d_mu: n_nodes), q (d_q: n_q) and η (d_eta), by a reverse sweep
  * of the same DP.  The reference obtains this with ForwardDiff

@@ -69,6 +69,7 @@ def lib():
         L.whale_batch_launches.argtypes = [vp, C.c_int32, i64p]
         L.whale_logpdf.argtypes = [vp, C.c_int32, C.c_int32, dblp, dblp]
         L.whale_logpdf_grad.argtypes = [vp, C.c_int32, C.c_int32, dblp, dblp, dblp, dblp, dblp, dblp]
+        L.whale_logpdf_noalloc.argtypes = [vp, C.c_int32, vp, vp]
         L.whale_download_ell.argtypes = [vp, C.c_int32, dblp, C.c_int64, i64p]
         _lib = L
     return _lib

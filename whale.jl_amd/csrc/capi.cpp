@@ -98,6 +98,7 @@ struct whale_ctx {
   hipEvent_t ev[6] = {};
   hipEvent_t user[16] = {};
   double last_ms[5] = {0, 0, 0, 0, 0};
+  unsigned char* scratch = nullptr;  // this device's copy of the fixed workspace (whale_logpdf_noalloc)
 };
 
 struct whale_model {
@@ -215,14 +216,14 @@ struct whale_batch {
     return W;
   }
   // the FP64 region used for families recomputed after an FP32 pass
-  WorkRaw work_fallback() const {
-    WorkRaw W = work();
-    auto off = [](uint64_t n) { return ((4 * n + 7) / 8) * 8; };
-    W.ell = (char*)ell.p + off(plan.ell_cells);
-    W.coef = (char*)coef.p + off(n_rec);
-    W.ibar = (char*)ibar.p + off(plan.ibar_cells);
+  WorkRaw work_fallback() const { return fallback_of(work()); }
+  WorkRaw fallback_of(WorkRaw W) const {
+    W.ell = (char*)W.ell + fp32_bytes(plan.ell_cells);
+    W.coef = (char*)W.coef + fp32_bytes(n_rec);
+    W.ibar = (char*)W.ibar + fp32_bytes(plan.ibar_cells);
     return W;
   }
+  static uint64_t fp32_bytes(uint64_t n) { return ((4 * n + 7) / 8) * 8; }
 };
 
 extern "C" {
@@ -255,6 +256,11 @@ int whale_ctx_create(int device, whale_ctx** out) {
   }
   for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
   for (auto& e : c->user) HIPCHK(hipEventCreate(&e));
+  {  // the device address of this GPU's fixed workspace, with the device current
+    void* p = nullptr;
+    HIPCHK(scratch_address(device, &p));
+    c->scratch = static_cast<unsigned char*>(p);
+  }
   *out = c.release();
   return WHALE_OK;
 }
@@ -910,13 +916,16 @@ void drop_graphs(whale_batch* b) {
 
 // All device work of one evaluation on the context's stream (capturable: no
 // memset nodes, no host synchronisation).  Timing events only when `timed`.
-int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all, bool timed) {
+// `ws`, when given, replaces the batch's workspace (and `ws_out` the model's
+// result buffer); the results then stay on the device.
+int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all, bool timed,
+            const WorkRaw* ws = nullptr, double* ws_out = nullptr) {
   whale_model* m = b->model;
   whale_ctx* c = m->ctx;
   hipStream_t st = c->stream;
   const BatchDev B = b->dev();
   const ModelDev M = m->dev();
-  const WorkRaw W = b->work();
+  const WorkRaw W = ws ? *ws : b->work();
   const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
   HIPCHK(launch_fill_u32(st, W.ticket, 0u, kTickets));
@@ -930,7 +939,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   // families the FP32 pass could not represent, recomputed in FP64 with per-clade exponents
   BatchDev Bfb = B;
   Bfb.order = W.fb + 1;
-  WorkRaw Wfb = b->work_fallback();
+  WorkRaw Wfb = b->fallback_of(W);
   // The size classes run on three streams at once (the context's stream and
   // two side streams), each taking classes largest first: a class's launch
   // holds a compute unit's LDS until its slowest family ends, and the classes
@@ -1037,7 +1046,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     HIPCHK(hipEventRecord(c->fork[6], st));
     HIPCHK(hipStreamWaitEvent(c->side2, c->fork[6], 0));
     const int rmode = fp32 ? WHALE_FP64_SCALED : mode;
-    WorkRaw Wr = fp32 ? b->work_fallback() : W;
+    WorkRaw Wr = fp32 ? b->fallback_of(W) : W;
     if (Wr.gws) Wr.gws += b->gws_main;  // its own stage-buffer region: it runs beside the other launches
     Wr.ticket = W.ticket + 2 * whale_batch::kClasses + 2;
     HIPCHK(launch_forward(rmode, store_all, b->cfg_route[mode][0], c->side2, B, M, Wr, -1, b->route_lo, nf));
@@ -1079,10 +1088,11 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   }
   if (routed) HIPCHK(hipStreamWaitEvent(st, c->fork[7], 0));
   if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
-  double* out = m->out.as<double>();
+  double* out = ws_out ? ws_out : m->out.as<double>();
   HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
                      m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out));
   if (timed) HIPCHK(hipEventRecord(c->ev[4], st));
+  if (ws) return WHALE_OK;
   const int n = m->h.n, nw = m->h.n_wgd;
   HIPCHK(hipMemcpyAsync(b->pin_out, out, sizeof(double) * (grad ? 2 + 2 * n + nw : 1), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(b->pin_fam, W.fam_ll, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
@@ -1174,6 +1184,58 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
 
 int whale_logpdf(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total) {
   return run(b, mode, launch, false, out_family, out_total, nullptr, nullptr, nullptr, nullptr);
+}
+
+static_assert(kScratchBytes == WHALE_NOALLOC_SCRATCH_BYTES, "fixed workspace size of the C ABI");
+
+int whale_logpdf_noalloc(whale_batch* b, int32_t mode, double* d_family, double* d_total) {
+  if (!b || !d_family || !d_total) return fail(WHALE_EINVAL, "whale_logpdf_noalloc: NULL argument");
+  if (mode < 0 || mode > 2) return fail(WHALE_EINVAL, "unknown mode");
+  whale_model* m = b->model;
+  whale_ctx* c = m->ctx;
+  if (!m->has_params) return fail(WHALE_ESTATE, "parameters not set (whale_model_set_params)");
+  if (!c->scratch) return fail(WHALE_ESTATE, "no fixed workspace on this device");
+  // The workspace of a logL evaluation without stored rows, carved from the
+  // fixed array: ℓ (FP32 values followed by the FP64 region of families an
+  // FP32 pass recomputes, or FP64 values), per-clade exponents, tickets,
+  // flags, the list of recomputed families and the stage buffers of
+  // global-memory launches.  Per-record coefficients, ī and gradient
+  // accumulators are not touched without stored rows or a gradient.
+  const PlanHost& P = b->plan;
+  auto al = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+  const uint64_t cells = P.ell_cells;
+  const uint64_t ell_b = mode == WHALE_FP32_SCALED ? whale_batch::fp32_bytes(cells) + 8 * cells : 8 * cells;
+  uint64_t off = 0;
+  const uint64_t o_ell = off;
+  off += al(std::max<uint64_t>(16, ell_b));
+  const uint64_t o_E = off;
+  off += al(4 * std::max<uint64_t>(1, (uint64_t)P.n_clades));
+  const uint64_t o_ticket = off;
+  off += al(4 * kTickets);
+  const uint64_t o_flag = off;
+  off += al(4 * (uint64_t)b->n_fam);
+  const uint64_t o_fb = off;
+  off += al(4 * ((uint64_t)b->n_fam + 1));
+  const uint64_t o_gws = off;
+  off += al(b->gws.n);
+  if (off > kScratchBytes)
+    return fail(WHALE_EINVAL, "whale_logpdf_noalloc: the batch needs " + std::to_string(off) +
+                                  " bytes of workspace, more than the fixed " + std::to_string(kScratchBytes) +
+                                  " bytes; evaluate it with whale_logpdf");
+  unsigned char* base = c->scratch;
+  WorkRaw W{};
+  W.ell = base + o_ell;
+  W.coef = base + o_ell;   // not read or written without stored rows
+  W.ibar = base + o_ell;   // not read or written without a gradient
+  W.acc = nullptr;         // not read or written without a gradient
+  W.E = reinterpret_cast<int32_t*>(base + o_E);
+  W.ticket = reinterpret_cast<uint32_t*>(base + o_ticket);
+  W.flag = reinterpret_cast<uint32_t*>(base + o_flag);
+  W.fb = reinterpret_cast<uint32_t*>(base + o_fb);
+  W.gws = b->gws.n ? base + o_gws : nullptr;
+  W.fam_ll = d_family;
+  HIPCHK(hipSetDevice(c->device));
+  return enqueue(b, mode, WHALE_LAUNCH_BATCHED, false, false, false, &W, d_total);
 }
 
 int whale_logpdf_grad(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total,

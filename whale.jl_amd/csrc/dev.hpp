@@ -143,4 +143,11 @@ hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int n);
 hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n);
 
+// The fixed per-GPU workspace of whale_logpdf_noalloc (a file-scope device
+// array in kernels.hip) and its device address on `device`, which must be the
+// current device; found once per device and kept.
+constexpr size_t kScratchBytes = 64ull << 20;
+constexpr int kMaxDevices = 64;
+hipError_t scratch_address(int device, void** out);
+
 }  // namespace whale

@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "dev.hpp"
 #include "../../include/whale_amd.h"
@@ -2019,6 +2020,34 @@ hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n) {
   int grid = (int)std::min<size_t>(1024, (n + 255) / 256 + 1);
   hipLaunchKernelGGL(k_fill_f64, dim3(grid), dim3(256), 0, st, p, v, n);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// fixed workspace of the evaluation that allocates nothing
+// ---------------------------------------------------------------------------
+// Every GPU the library runs on holds one copy of this array until the
+// process ends.  Host code never passes its name to a kernel (on the host it
+// is a stand-in, not a device address): the device address of each GPU's copy
+// comes from hipGetSymbolAddress, once per device, and only that is used.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+hipError_t scratch_address(int device, void** out) {
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes `device` current
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *out = addr[device];
+  return hipSuccess;
 }
 
 }  // namespace whale
