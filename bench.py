@@ -75,7 +75,7 @@ def copy_probe(nbytes=1 << 28):
     log(f"copy probe: {nbytes} bytes device-to-device")
 
 
-def sampler(w, synth, workload="config5", n_points=256, n_families=None):
+def sampler(w, synth, workload="config5", n_points=256, n_families=None, tune_cache=None, tune=True):
     """Sampler-style loop (config5: 20 taxa + 5 WGDs, 5,000 families):
     device-resident CCDs, parameter points drawn as log-normal perturbations
     of the truth, each evaluated (logL + gradient, FP32-scaled, batched)
@@ -85,10 +85,16 @@ def sampler(w, synth, workload="config5", n_points=256, n_families=None):
     wl = synth.workload(workload, n_families=n_families)
     model = w.WhaleModel(wl.rates, w.readnw(wl.newick), wl.dt)
     ccds = w.ccds_from_texts(wl.texts, model)
-    n_fam = w.device_batch(model, ccds).n
+    batch = w.device_batch(model, ccds)
+    n_fam = batch.n
     points = synth.perturbed(wl.rates, n_points)
     models = [model(r) for r in points]
     log(f"sampler workload {workload}: {n_fam} families ready in {time.time() - t0:.1f}s")
+    tuned = None
+    if tune:
+        model._sync()
+        tuned = batch.autotune("fp32", gradient=True, reps=5, cache=tune_cache)
+        log(f"sampler autotune: {tuned}")
     for m in models[:3]:
         w.gradient(m, ccds, mode="fp32")
     lat = []
@@ -102,7 +108,8 @@ def sampler(w, synth, workload="config5", n_points=256, n_families=None):
     out = {"workload": f"{wl.name}: {n_fam} families, {len(points)} parameter points back to back, "
                        "logL+gradient FP32-scaled, device-resident CCDs",
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-           "evals_per_s": len(points) / total, "families_per_s": n_fam * len(points) / total}
+           "evals_per_s": len(points) / total, "families_per_s": n_fam * len(points) / total,
+           "tuning": tuned}
     log(f"sampler: p50 {out['p50_ms']:.3f} ms, p99 {out['p99_ms']:.3f} ms, {out['evals_per_s']:.1f} evals/s")
     return out
 
@@ -185,6 +192,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--families", type=int, default=None)
     ap.add_argument("--no-sampler", action="store_true", help="skip the config5 sampler-loop leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the hand-picked launch shape (no autotuning)")
+    ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "whale_amd", "tuning.txt"),
+                    help="autotuner cache file (empty: search every time)")
     ap.add_argument("--sampler-points", type=int, default=256)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device-to-device copy of 256 MiB (calibrates rocprofv3 byte counters)")
@@ -247,6 +257,16 @@ def main():
     log(f"logL fp64s {v64s:.6f} fp64 {v64:.6f} fp32 {v32:.6f}; fp32 rel err family max {rel_fam:.2e} "
         f"total {rel_tot:.2e} grad {gerr:.2e}; fp64-plain underflowed families {n_underflow}; "
         f"fp32 families recomputed in fp64 {n_fallback}")
+
+    # ---- launch shape: the autotuner's search on this batch (or its cached result)
+    tune = None
+    if not args.no_tune:
+        cache = args.tune_cache or None
+        if cache:
+            os.makedirs(os.path.dirname(cache), exist_ok=True)
+        tune = batch.autotune("fp32", gradient=True, reps=5, cache=cache)
+        tune["tuning_of"] = "class LDS bounds (bytes), stream layout, waves per CU; ms are host-timed graph replays"
+        log(f"autotune: {tune}")
 
     # ---- timed: batched FP32-scaled logL + gradient (captured HIP graph replays)
     for _ in range(args.warmup):
@@ -311,7 +331,8 @@ def main():
     # ---- the sampler-style loop (config5), its own workload
     samp = None
     if not args.no_sampler:
-        samp = sampler(w, synth, n_points=args.sampler_points)
+        samp = sampler(w, synth, n_points=args.sampler_points, tune_cache=args.tune_cache or None,
+                       tune=not args.no_tune)
 
     # ---- CPU baseline: the oracle (C++ restatement) on a bounded sample
     cpu = cpu2 = calib = None
@@ -356,6 +377,7 @@ def main():
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "speedup_vs_cpu_2_threads": (value / cpu2["value"]) if cpu2 else None,
         "sampler": samp,
+        "tuning": tune,
         "fp64_plain_batched_ms": fp64_batched_ms,
         "kernel_ms": {"tables": kern[0], "forward": kern[1], "backward": kern[2], "tail": kern[3]},
     }

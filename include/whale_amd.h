@@ -172,6 +172,26 @@ int whale_batch_hbm_families(const whale_batch* b, int32_t* n);
  * counterpart (testing and tuning). */
 int whale_batch_set_lds_limit(whale_batch* b, int64_t bytes);
 
+/* Launch-shape autotuner (no reference counterpart): searches the batch's
+ * size-class LDS bounds, the spread of the classes over the streams (one of
+ * six layouts) and the waves per compute unit a launch aims for, timing
+ * captured evaluations of `mode` (logL + gradient when `gradient` != 0) on
+ * the batch itself, `reps` timed runs per candidate (median; a change is kept
+ * when at least 1 % faster).  The result is applied to the batch and, when
+ * cache_path is not NULL, appended to that text file under a key of the
+ * device, the arithmetic and the workload's shape; a later call whose key is
+ * in the file applies the cached shape without searching.  Parameters must be
+ * set; the evaluations overwrite the batch's last results.
+ *   out8 (may be NULL): [5 class bounds in bytes, layout, waves per CU,
+ *                        candidates timed (0 when taken from the cache)]
+ *   out_ms2 (may be NULL): [ms per evaluation before, after] */
+int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t reps, const char* cache_path,
+                         int64_t* out8, double* out_ms2);
+/* Applies / reads a launch shape: [5 class bounds in bytes (increasing,
+ * >= 1024), layout 0..5, waves per CU 1..32].  No reference counterpart. */
+int whale_batch_set_tuning(whale_batch* b, const int64_t* in7);
+int whale_batch_get_tuning(const whale_batch* b, int64_t* out7);
+
 /* logpdf of every family (unconditioned, src/core.jl:31-43) into
  * out_family[n_families] (may be NULL) and the conditioned batch value
  * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */

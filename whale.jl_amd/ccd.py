@@ -187,6 +187,26 @@ class DeviceBatch:
         limit); families above it run with their stage buffers in global memory."""
         check(_lib.lib().whale_batch_set_lds_limit(self.h, int(nbytes)), "whale_batch_set_lds_limit")
 
+    def autotune(self, mode: str = "fp32", gradient: bool = True, reps: int = 5, cache: Optional[str] = None) -> dict:
+        """Searches this batch's launch shape (whale_batch_autotune) and applies it;
+        `cache` is a text file of earlier results, keyed by device and workload."""
+        o = np.zeros(8, np.int64)
+        ms = np.zeros(2)
+        check(_lib.lib().whale_batch_autotune(self.h, _lib.MODES[mode], int(gradient), int(reps),
+                                              cache.encode() if cache else None, ptr(o, i64p), ptr(ms, dblp)),
+              "whale_batch_autotune")
+        return dict(class_bounds=[int(x) for x in o[:5]], layout=int(o[5]), waves_per_cu=int(o[6]),
+                    candidates_timed=int(o[7]), ms_before=float(ms[0]), ms_after=float(ms[1]))
+
+    def tuning(self) -> list:
+        o = np.zeros(7, np.int64)
+        check(_lib.lib().whale_batch_get_tuning(self.h, ptr(o, i64p)), "whale_batch_get_tuning")
+        return [int(x) for x in o]
+
+    def set_tuning(self, t: Sequence[int]):
+        a = np.asarray(t, np.int64)
+        check(_lib.lib().whale_batch_set_tuning(self.h, ptr(a, i64p)), "whale_batch_set_tuning")
+
     def launches(self, mode: str = "fp32") -> list:
         """Launch shape of each batched kernel: forward and reverse sweep per size class."""
         ncls = _lib.SIZE_CLASSES

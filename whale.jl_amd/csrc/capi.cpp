@@ -3,8 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <memory>
 #include <string>
 #include <thread>
@@ -61,6 +65,27 @@ hipError_t upload(DBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 struct whale_batch;
 namespace {
+// The searchable launch shape of a batch.
+constexpr int kLayouts = 6;
+struct Tuning {
+  uint32_t bounds[5] = {10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024, 48 * 1024};  // class LDS bounds (bytes)
+  int layout = 0;      // index into kLayoutOrder / kLayoutLane
+  int waves_cap = 8;   // resident waves per compute unit a launch aims for
+};
+// Class launch orders (largest first within a stream) and the stream of each
+// class, for the candidate layouts.  Layout 0 was chosen by hand among a
+// dozen on the 10,000-family workload; the others are what the search tries.
+constexpr int kLayoutOrder[kLayouts][6] = {
+    {5, 4, 1, 3, 2, 0}, {5, 4, 3, 2, 1, 0}, {5, 4, 3, 2, 1, 0},
+    {5, 4, 3, 2, 1, 0}, {5, 4, 3, 2, 1, 0}, {5, 4, 3, 2, 1, 0}};
+constexpr int kLayoutLane[kLayouts][6] = {  // stream of class 0..5
+    {2, 2, 1, 0, 1, 0},   // 5 then 3 | 4 then 2 | 1 then 0
+    {0, 1, 2, 0, 1, 2},   // 5, 2 | 4, 1 | 3, 0
+    {0, 0, 1, 1, 2, 2},   // 5, 4 | 3, 2 | 1, 0
+    {1, 2, 0, 2, 1, 0},   // 5, 2 | 4, 0 | 3, 1
+    {2, 1, 0, 0, 1, 2},   // 5, 0 | 4, 1 | 3, 2
+    {1, 1, 2, 1, 2, 0}};  // 5 alone | 4, 2 | 3, 1, 0
+
 // family tickets: one per class and direction, two for the FP64 pass after an
 // FP32 pass, two for the families routed to FP64 from the start, two for the
 // families whose stages do not fit LDS
@@ -155,6 +180,10 @@ struct whale_batch {
   uint32_t hbm_lo = 0, hbm_hi = 0;
   size_t lds_limit = 0;  // 0: the device's LDS per workgroup
   LaunchCfg cfg_hbm[3][2];
+  // Launch-shape parameters (whale_batch_autotune searches them): the LDS
+  // bounds of the size classes, how the classes are spread over the three
+  // streams, and the most waves per compute unit a launch plans for.
+  Tuning tune;
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   int last_mode = -1;
@@ -723,7 +752,8 @@ namespace {
 constexpr int kWavesPerCU = 8;
 constexpr int kMaxFPW = 8;
 
-bool plan_launch(const whale_model* m, bool fwd, int mode, size_t wave, uint64_t nfam, bool multi, LaunchCfg* out) {
+bool plan_launch(const whale_model* m, bool fwd, int mode, size_t wave, uint64_t nfam, bool multi, LaunchCfg* out,
+                 int waves_cap = kWavesPerCU) {
   wave = align16(std::max<size_t>(wave, 16));
   const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   auto shared_of = [&](bool tabl) {
@@ -739,7 +769,7 @@ bool plan_launch(const whale_model* m, bool fwd, int mode, size_t wave, uint64_t
     for (int fpw = 1; fpw <= (multi ? kMaxFPW : 1); ++fpw) {
       const size_t lds = shared_of(tabl) + (size_t)fpw * wave;
       if (lds > m->ctx->lds_wg) break;
-      const int wg = (int)std::min<size_t>(m->ctx->lds_cu / lds, (size_t)(kWavesPerCU / fpw));
+      const int wg = (int)std::min<size_t>(m->ctx->lds_cu / lds, (size_t)(waves_cap / fpw));
       if (wg * fpw > best_waves) {
         best_fpw = fpw;
         best_waves = wg * fpw;
@@ -773,8 +803,8 @@ bool plan_families(const whale_batch* b, bool fwd, int mode, const uint32_t* fam
     without = std::max(without, fam_lds(d, mode, fwd, false));
   }
   LaunchCfg a, c;
-  const bool oa = !force_gws && plan_launch(b->model, fwd, mode, with, grid_fams, multi, &a);
-  const bool oc = !force_gws && plan_launch(b->model, fwd, mode, without, grid_fams, multi, &c);
+  const bool oa = !force_gws && plan_launch(b->model, fwd, mode, with, grid_fams, multi, &a, b->tune.waves_cap);
+  const bool oc = !force_gws && plan_launch(b->model, fwd, mode, without, grid_fams, multi, &c, b->tune.waves_cap);
   if (oa && (!oc || a.waves_per_cu >= c.waves_per_cu)) *out = a;
   else if (oc) *out = c;
   if (oa || oc) return true;
@@ -841,7 +871,7 @@ int plan_batch_launches(whale_batch* b) {
 // routed to FP64 last.  Then the launch shapes.
 int assign_classes(whale_batch* b) {
   const PlanHost& P = b->plan;
-  const size_t bounds[whale_batch::kClasses - 1] = {10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024, 48 * 1024};
+  const uint32_t* bounds = b->tune.bounds;
   std::vector<uint32_t> cls[whale_batch::kClasses], rt, hbm;
   for (uint32_t f : P.cost_order) {
     if (b->routed[f]) {
@@ -948,9 +978,8 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     constexpr int kLanes = 3;
     hipStream_t lanes[kLanes] = {st, c->side, c->side3};
     // classes in launch order and their streams
-    static const int kOrder[2][whale_batch::kClasses] = {{5, 4, 1, 3, 2, 0}, {5, 4, 1, 3, 2, 0}};
-    static const int kLane[2][whale_batch::kClasses] = {{2, 2, 1, 0, 1, 0}, {2, 2, 1, 0, 1, 0}};  // by class
-    auto lane_of = [&](int k) { return kLane[dir][k]; };
+    const int* kOrder = kLayoutOrder[b->tune.layout];
+    auto lane_of = [&](int k) { return kLayoutLane[b->tune.layout][k]; };
     bool used[kLanes] = {};
     for (int k = 0; k < whale_batch::kClasses; ++k)
       if (b->cls_hi[k] > b->cls_lo[k]) used[lane_of(k)] = true;
@@ -965,7 +994,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
       else HIPCHK(launch_backward(mode, C, st, B, M, Wh, -1, b->hbm_lo, b->hbm_hi));
     }
     for (int j = 0; j < whale_batch::kClasses; ++j) {
-      const int k = kOrder[dir][j];
+      const int k = kOrder[j];
       if (b->cls_hi[k] == b->cls_lo[k]) continue;
       hipStream_t sk = lanes[lane_of(k)];
       WorkRaw Wk = W;
@@ -989,8 +1018,8 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   auto interleaved = [&]() -> int {
     constexpr int kLanes = 3;
     hipStream_t lanes[kLanes] = {st, c->side, c->side3};
-    static const int kOrder[whale_batch::kClasses] = {5, 4, 1, 3, 2, 0};
-    static const int kLane[whale_batch::kClasses] = {2, 2, 1, 0, 1, 0};  // by class
+    const int* kOrder = kLayoutOrder[b->tune.layout];
+    const int* kLane = kLayoutLane[b->tune.layout];
     bool used[kLanes] = {};
     for (int k = 0; k < whale_batch::kClasses; ++k)
       if (b->cls_hi[k] > b->cls_lo[k]) used[kLane[k]] = true;
@@ -1181,6 +1210,209 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
 }
 
 }  // namespace
+
+namespace {
+
+// Re-plans the batch for its current tuning and drops captured evaluations.
+int replan(whale_batch* b) {
+  int rc = assign_classes(b);
+  if (rc != WHALE_OK) return rc;
+  if ((rc = ensure_gws(b, nullptr)) != WHALE_OK) return rc;
+  HIPCHK(hipMemcpy(b->order.p, b->order_host.data(), sizeof(uint32_t) * b->n_fam, hipMemcpyHostToDevice));
+  drop_graphs(b);
+  return WHALE_OK;
+}
+
+// Median wall time (ms) of `reps` captured evaluations after one untimed one
+// (which captures the graph), as a caller sees them: launch to results on the host.
+int time_evals(whale_batch* b, int mode, bool grad, int reps, double* ms) {
+  const whale_model* m = b->model;
+  std::vector<double> fam(b->n_fam), dl(m->h.n), dm(m->h.n), dq(std::max(1, m->h.n_wgd));
+  double tot = 0, de = 0;
+  std::vector<double> t;
+  for (int i = 0; i <= reps; ++i) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = run(b, mode, WHALE_LAUNCH_BATCHED, grad, fam.data(), &tot, dl.data(), dm.data(), dq.data(), &de);
+    if (rc != WHALE_OK) return rc;
+    const auto t1 = std::chrono::steady_clock::now();
+    if (i > 0) t.push_back(std::chrono::duration<double, std::milli>(t1 - t0).count());
+  }
+  std::sort(t.begin(), t.end());
+  *ms = t[t.size() / 2];
+  return WHALE_OK;
+}
+
+// The workload a tuning applies to: device, arithmetic, gradient or not, and
+// every family's shape as the planner sees it.
+std::string tune_key(const whale_batch* b, int mode, bool grad) {
+  const whale_model* m = b->model;
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xff;
+      h *= 1099511628211ull;
+    }
+  };
+  mix(b->n_fam);
+  mix((uint64_t)m->h.tab_rows);
+  mix((uint64_t)m->h.n);
+  for (const FamDev& d : b->plan.fam) {
+    mix(d.n_clades);
+    mix(d.max_items);
+    mix(d.max_recs);
+    mix(d.root_recs);
+    mix(d.n_stages);
+  }
+  hipDeviceProp_t prop;
+  std::string arch = "gpu";
+  if (hipGetDeviceProperties(&prop, m->ctx->device) == hipSuccess) arch = prop.gcnArchName;
+  for (char& ch : arch)
+    if (ch == ' ' || ch == ':') ch = '_';
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s/cu%d/lds%zu/mode%d/grad%d/%016llx", arch.c_str(), m->ctx->n_cu, m->ctx->lds_wg, mode,
+           grad ? 1 : 0, (unsigned long long)h);
+  return buf;
+}
+
+bool valid_tuning(const Tuning& t) {
+  if (t.layout < 0 || t.layout >= kLayouts || t.waves_cap < 1 || t.waves_cap > 32) return false;
+  for (int i = 0; i < 5; ++i) {
+    if (t.bounds[i] < 1024) return false;
+    if (i > 0 && t.bounds[i] <= t.bounds[i - 1]) return false;
+  }
+  return true;
+}
+
+bool cache_lookup(const char* path, const std::string& key, Tuning* t) {
+  if (!path || !*path) return false;
+  std::ifstream f(path);
+  std::string line;
+  bool found = false;
+  while (std::getline(f, line)) {  // the last line for a key wins
+    std::istringstream is(line);
+    std::string k;
+    Tuning u;
+    double ms = 0;
+    if (!(is >> k) || k != key) continue;
+    for (auto& x : u.bounds) is >> x;
+    is >> u.layout >> u.waves_cap >> ms;
+    if (is && valid_tuning(u)) {
+      *t = u;
+      found = true;
+    }
+  }
+  return found;
+}
+
+void cache_store(const char* path, const std::string& key, const Tuning& t, double ms) {
+  if (!path || !*path) return;
+  std::ofstream f(path, std::ios::app);
+  if (!f) return;  // a cache that cannot be written only costs a search next time
+  f << key;
+  for (auto x : t.bounds) f << ' ' << x;
+  f << ' ' << t.layout << ' ' << t.waves_cap << ' ' << ms << '\n';
+}
+
+}  // namespace
+
+int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t reps, const char* cache_path,
+                         int64_t* out8, double* out_ms2) {
+  if (!b) return fail(WHALE_EINVAL, "NULL argument");
+  if (mode < 0 || mode > 2) return fail(WHALE_EINVAL, "unknown mode");
+  if (!b->model->has_params) return fail(WHALE_ESTATE, "parameters not set (whale_model_set_params)");
+  if (b->model->ctx->timing) return fail(WHALE_ESTATE, "per-kernel timing is on; the search times graph replays");
+  reps = std::max(1, std::min(reps, 50));
+  const bool grad = gradient != 0;
+  HIPCHK(hipSetDevice(b->device));
+  const std::string key = tune_key(b, mode, grad);
+  const Tuning start = b->tune;
+  int rc;
+  double t_start = 0, t_best = 0;
+  int evals = 0;
+  Tuning best;
+  if (cache_lookup(cache_path, key, &best)) {
+    b->tune = best;
+    if ((rc = replan(b)) != WHALE_OK) return rc;
+    if ((rc = time_evals(b, mode, grad, reps, &t_best)) != WHALE_OK) return rc;
+    t_start = t_best;
+  } else {
+    // coordinate search from the current shape, accepting a change only when
+    // it is at least 1 % faster (run-to-run noise is below that)
+    auto timed = [&](const Tuning& t, double* ms) -> int {
+      b->tune = t;
+      int r = replan(b);
+      if (r == WHALE_OK) r = time_evals(b, mode, grad, reps, ms);
+      ++evals;
+      return r;
+    };
+    best = start;
+    if ((rc = timed(best, &t_start)) != WHALE_OK) return rc;
+    t_best = t_start;
+    auto consider = [&](const Tuning& t) -> int {
+      if (!valid_tuning(t)) return WHALE_OK;
+      double ms = 0;
+      const int r = timed(t, &ms);
+      if (r != WHALE_OK) return r;
+      if (ms < 0.99 * t_best) {
+        best = t;
+        t_best = ms;
+      }
+      return WHALE_OK;
+    };
+    for (int l = 0; l < kLayouts; ++l) {
+      if (l == best.layout) continue;
+      Tuning t = best;
+      t.layout = l;
+      if ((rc = consider(t)) != WHALE_OK) return rc;
+    }
+    for (int cap : {4, 6, 12, 16}) {
+      Tuning t = best;
+      t.waves_cap = cap;
+      if ((rc = consider(t)) != WHALE_OK) return rc;
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (int i = 0; i < 5; ++i)
+        for (double f : {0.75, 0.9, 1.12, 1.33}) {
+          Tuning t = best;
+          t.bounds[i] = (uint32_t)((double)best.bounds[i] * f) & ~255u;
+          if ((rc = consider(t)) != WHALE_OK) return rc;
+        }
+    b->tune = best;
+    if ((rc = replan(b)) != WHALE_OK) return rc;
+    cache_store(cache_path, key, best, t_best);
+  }
+  if (out8) {
+    for (int i = 0; i < 5; ++i) out8[i] = best.bounds[i];
+    out8[5] = best.layout;
+    out8[6] = best.waves_cap;
+    out8[7] = evals;
+  }
+  if (out_ms2) {
+    out_ms2[0] = t_start;
+    out_ms2[1] = t_best;
+  }
+  return WHALE_OK;
+}
+
+int whale_batch_set_tuning(whale_batch* b, const int64_t* in7) {
+  if (!b || !in7) return fail(WHALE_EINVAL, "NULL argument");
+  Tuning t;
+  for (int i = 0; i < 5; ++i) t.bounds[i] = (uint32_t)in7[i];
+  t.layout = (int)in7[5];
+  t.waves_cap = (int)in7[6];
+  if (!valid_tuning(t)) return fail(WHALE_EINVAL, "invalid tuning: increasing class bounds >= 1024, layout 0..5, 1..32 waves");
+  HIPCHK(hipSetDevice(b->device));
+  b->tune = t;
+  return replan(b);
+}
+
+int whale_batch_get_tuning(const whale_batch* b, int64_t* out7) {
+  if (!b || !out7) return fail(WHALE_EINVAL, "NULL argument");
+  for (int i = 0; i < 5; ++i) out7[i] = b->tune.bounds[i];
+  out7[5] = b->tune.layout;
+  out7[6] = b->tune.waves_cap;
+  return WHALE_OK;
+}
 
 int whale_logpdf(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total) {
   return run(b, mode, launch, false, out_family, out_total, nullptr, nullptr, nullptr, nullptr);
