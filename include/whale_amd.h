@@ -231,6 +231,24 @@ int whale_logpdf_grad(whale_batch* b, int32_t mode, int32_t launch, double* out_
  * small. */
 int whale_download_ell(whale_batch* b, int32_t family, double* out, int64_t cap, int64_t* n_out);
 
+/* Stochastic backtracking of reconciled trees from the ℓ the last call
+ * stored (backtrack(wm, ccd) after logpdf!(wm, ccd), src/track.jl:188-414),
+ * on the device, one lane per (family, sample): n_samples trees for each of
+ * the families [family, family + n_families).  Events are drawn with the
+ * reference's weights and in its order; the uniform numbers come from a
+ * counter-based generator of (seed, family, sample), so a call is
+ * reproducible.  Tree s of family family + i is
+ *   counts[i * n_samples + s]   its node count, or -1 if it needs more than cap
+ *   nodes[((i * n_samples + s) * cap + k) * 4 + 0..3] for k < count:
+ *       clade (0-based, -1 for a loss leaf), species node (id - 1), slice row t
+ *       (1-based, as SliceState.t; 0 for a loss), parent node (-1 for the root)
+ * in the order the nodes are created (a node is created when the walk enters
+ * a new (clade, branch) pair, src/track.jl:172-180).  Needs the last call to
+ * have stored ℓ (WHALE_KEEP_ELL or the gradient); families whose likelihood
+ * is zero are refused. */
+int whale_backtrack(whale_batch* b, int32_t family, int32_t n_families, int32_t n_samples, uint64_t seed,
+                    int32_t cap, int32_t* counts, int32_t* nodes);
+
 /* HIP events on the context's stream for callers' own timing: record event
  * `slot` (0..15) now; elapsed ms between two recorded slots (synchronises on
  * the later one).  No reference counterpart. */

@@ -65,6 +65,8 @@ def lib():
                                C.c_int, dp, dp]
         L.orc_tables.argtypes = [C.c_void_p, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp, dp]
         L.orc_getslice.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_backtrack.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int,
+                                    C.c_uint64, C.c_int, C.c_int, ip, ip]
         L.orc_ell.restype = C.c_long
         L.orc_ell.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int,
                               dp, C.c_long]
@@ -219,6 +221,20 @@ class OracleModel:
         e, f, s = np.zeros(nrows), np.zeros(nrows), np.zeros(nrows)
         n = lib().orc_tables(self.h, kind, _dp(th), nl, nq, _dp(p), len(p), node_id, _dp(e), _dp(f), _dp(s))
         return e[:n], f[:n], s[:n]
+
+    def backtrack(self, ccds: OracleCCDs, fam: int, rates, seed: int, n_samples: int, cap: int = 4096):
+        """Reconciled trees sampled by stochastic backtracking (src/track.jl:188-414)
+        from the oracle's ℓ: per sample an (n, 4) array of nodes in creation
+        order, [clade (0-based, -1 loss), species node (0-based), slice row, parent]."""
+        kind, th, nl, nq, p = rates.pack()
+        counts = np.zeros(n_samples, np.int32)
+        nodes = np.zeros(n_samples * cap * 4, np.int32)
+        rc = lib().orc_backtrack(self.h, ccds.h, fam, kind, _dp(th), nl, nq, _dp(p), len(p), seed, n_samples, cap,
+                                 _ip(counts), _ip(nodes))
+        if rc != 0:
+            raise ValueError(lib().orc_last_error().decode())
+        nodes = nodes.reshape(n_samples, cap, 4)
+        return [nodes[i, :counts[i]].copy() if counts[i] >= 0 else None for i in range(n_samples)]
 
     def ell(self, ccds: OracleCCDs, fam: int, rates) -> np.ndarray:
         kind, th, nl, nq, p = rates.pack()

@@ -51,6 +51,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <functional>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -945,6 +946,139 @@ long orc_ell(void* mp, void* cp, int fam, int kind, const double* theta, int nl,
   for (size_t e = 0; e < m.nodes.size(); ++e)
     for (double v : dp.l.a[e]) { if (k < cap) out[k] = v; ++k; }
   return k;
+}
+
+// Stochastic backtracking of one family (reference: src/track.jl:188-414)
+// from the double ℓ of the oracle's own DP: n_samples reconciled trees, each
+// as its nodes in creation order, 4 ints per node (0-based clade or -1 for a
+// loss, 0-based species node, slice row (1-based, as SliceState.t), parent
+// node or -1), at most `cap` nodes per sample (counts[s] = -1 past that).
+// The uniform draws come from a counter-based generator of (seed, family,
+// sample) and are drawn in the reference's order (depth first, one draw per
+// inter- or intra-branch step), so a device sampler that follows the same
+// order draws the same numbers.
+static double bt_uniform(uint64_t& st) {
+  st += 0x9E3779B97F4A7C15ull;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+int orc_backtrack(void* mp, void* cp, int fam, int kind, const double* theta, int nl, int nq, const double* p,
+                  int np, uint64_t seed, int n_samples, int cap, int* counts, int* nodes) {
+  try {
+    auto& m = ((OrcModel*)mp)->m;
+    auto& x = ((OrcCCDs*)cp)->v[fam];
+    auto r = make_rates<double>(kind, theta, nl, nq, p, np);
+    auto tb = build_tables(m, r);
+    DP<double> dp(m, x, tb, r);
+    dp.run();
+    const int root = m.root_id();
+    const int gam = (int)x.clades.size();
+    for (int smp = 0; smp < n_samples; ++smp) {
+      uint64_t st = seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(fam + 1)) ^ (0xD1B54A32D192ED03ull * (uint64_t)(smp + 1));
+      int* out = nodes + (size_t)smp * cap * 4;
+      int n = 0;
+      bool over = false;
+      auto add = [&](int g, int e, int t, int par) -> int {
+        if (n >= cap) { over = true; return -1; }
+        int* o = out + 4 * n;
+        o[0] = g - 1; o[1] = e - 1; o[2] = t; o[3] = par;
+        return n++;
+      };
+      struct St { int e, g, t; };
+      // bt(node, state): the recursion of backtrack!(b) with b.node = node
+      std::function<void(int, St)> bt;
+      auto go = [&](int node, St s) {  // (b::BackTracker)(newstate), then backtrack!
+        if (over) return;
+        const int* nd = out + 4 * node;
+        if (s.g - 1 != nd[0] || s.e - 1 != nd[1]) {
+          int c = add(s.g, s.e, s.t, node);
+          if (c < 0) return;
+          bt(c, s);
+        } else {
+          bt(node, s);
+        }
+      };
+      bt = [&](int node, St s) {
+        if (over) return;
+        if (s.g == 0) return;  // loss
+        const MNode& n_ = m.nd(s.e);
+        if (s.t == 1) {
+          if (n_.leaf) return;
+          double rr = bt_uniform(st) * dp.getl(s.e, s.g, 1);
+          const Clade& g = x.clades[s.g - 1];
+          std::vector<St> next;
+          auto pick = [&](double term, std::vector<St> nx) {
+            if (!next.empty()) return;
+            if (term > 0) { rr -= term; if (rr < 0) next = nx; }
+          };
+          // a term that ends the scan with rr still >= 0 (rounding): the last
+          // non-zero one is taken
+          std::vector<St> last;
+          auto note = [&](double term, std::vector<St> nx) { if (term > 0) last = nx; pick(term, nx); };
+          if (n_.root) {
+            const double eta = r.get(m, n_).eta, eps = dp.eps_top(s.e), xi = 1.0 - (1.0 - eta) * eps;
+            const double cA = xi * (1.0 - eta) / eta, cB = eta * (1.0 - eps) / (xi * xi);
+            const int f = n_.children[0], gg = n_.children[1];
+            const int tf = m.nd(f).k + 1, tg = m.nd(gg).k + 1;
+            for (auto& t3 : g.splits) {
+              note(t3.p * dp.getl(s.e, t3.g1, 1) * dp.getl(s.e, t3.g2, 1) * cA, {{s.e, t3.g1, 1}, {s.e, t3.g2, 1}});
+              note(t3.p * dp.getl(f, t3.g1) * dp.getl(gg, t3.g2) * cB, {{f, t3.g1, tf}, {gg, t3.g2, tg}});
+              note(t3.p * dp.getl(gg, t3.g1) * dp.getl(f, t3.g2) * cB, {{gg, t3.g1, tg}, {f, t3.g2, tf}});
+            }
+            note(dp.getl(f, s.g) * dp.eps_top(gg) * cB, {{f, s.g, tf}, {gg, 0, 0}});
+            note(dp.getl(gg, s.g) * dp.eps_top(f) * cB, {{gg, s.g, tg}, {f, 0, 0}});
+          } else if (n_.wgd) {
+            const double q = r.get(m, n_).q;
+            const int f = n_.children[0], tf = m.nd(f).k + 1;
+            note((1.0 - q + 2.0 * q * dp.eps_top(f)) * dp.getl(f, s.g), {{f, s.g, tf}});
+            for (auto& t3 : g.splits)
+              note(q * t3.p * dp.getl(f, t3.g1) * dp.getl(f, t3.g2), {{f, t3.g1, tf}, {f, t3.g2, tf}});
+          } else {
+            const int f = n_.children[0], gg = n_.children[1];
+            const int tf = m.nd(f).k + 1, tg = m.nd(gg).k + 1;
+            note(dp.getl(f, s.g) * dp.eps_top(gg), {{f, s.g, tf}, {gg, 0, 0}});
+            note(dp.getl(gg, s.g) * dp.eps_top(f), {{gg, s.g, tg}, {f, 0, 0}});
+            for (auto& t3 : g.splits) {
+              note(t3.p * dp.getl(f, t3.g1) * dp.getl(gg, t3.g2), {{f, t3.g1, tf}, {gg, t3.g2, tg}});
+              note(t3.p * dp.getl(gg, t3.g1) * dp.getl(f, t3.g2), {{gg, t3.g1, tg}, {f, t3.g2, tf}});
+            }
+          }
+          if (next.empty()) next = last;
+          for (auto& nx : next) go(node, nx);
+          return;
+        }
+        // within the branch
+        const Clade& g = x.clades[s.g - 1];
+        if (g.size == 1) return bt(node, {s.e, s.g, 1});
+        double rr = bt_uniform(st) * dp.getl(s.e, s.g, s.t);
+        const double stay = tb.phi[s.e - 1][s.t - 1] * dp.getl(s.e, s.g, s.t - 1);
+        rr -= stay;
+        if (rr < 0) return bt(node, {s.e, s.g, s.t - 1});
+        const double ps = tb.psi[s.e - 1][s.t - 1];
+        // the duplication where rr falls below 0, or (rounding) the last
+        // non-zero one; with no duplication weight at all, "no event"
+        std::vector<St> next;
+        for (auto& t3 : g.splits) {
+          const double term = t3.p * dp.getl(s.e, t3.g1, s.t - 1) * dp.getl(s.e, t3.g2, s.t - 1) * ps;
+          if (term > 0) {
+            next = {{s.e, t3.g1, s.t - 1}, {s.e, t3.g2, s.t - 1}};
+            rr -= term;
+            if (rr < 0) break;
+          }
+        }
+        if (next.empty()) return bt(node, {s.e, s.g, s.t - 1});
+        for (auto& nx : next) go(node, nx);
+      };
+      add(gam, root, 1, -1);
+      bt(0, {root, gam, 1});
+      counts[smp] = over ? -1 : n;
+    }
+    return 0;
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 
 }  // extern "C"

@@ -18,12 +18,14 @@ from .model import (WhaleModel, DLWGD, ConstantDLWGD, RootCondition, NonExtinctC
 from .ccd import CCD, CCDSet, read_ale, ccds_from_texts, DeviceBatch, device_batch
 from .core import logpdf, logpdf_, loglikelihood, logpdf_families, gradient
 from .mixture import MixtureModel, ModelArray, condition
+from .track import RecTree, backtrack, pair_posteriors, pair_labels
 
 __all__ = ["Node", "readnw", "insertnode", "getlca", "getleaves", "postwalk", "prewalk", "nwstr", "deepcopy_tree",
            "WhaleModel", "DLWGD", "ConstantDLWGD", "RootCondition", "NonExtinctCondition", "NoCondition",
            "SamplingCondition", "CCD", "CCDSet", "read_ale", "ccds_from_texts", "DeviceBatch", "device_batch",
            "logpdf", "logpdf_", "loglikelihood", "logpdf_families", "gradient", "context",
-           "MixtureModel", "ModelArray", "condition", "NowhereExtinctCondition"]
+           "MixtureModel", "ModelArray", "condition", "NowhereExtinctCondition", "RecTree", "backtrack",
+           "pair_posteriors", "pair_labels"]
 
 EXTREE = ("((MPOL:4.752,PPAT:4.752):0.292,(SMOE:4.457,(((OSAT:1.555,(ATHA:0.5548,CPAP:0.5548):1.0002):0"
           ".738,ATRI:2.293):1.225,(GBIL:3.178,PABI:3.178):0.34):0.939):0.587);")

@@ -74,6 +74,7 @@ def lib():
         L.whale_logpdf_grad.argtypes = [vp, C.c_int32, C.c_int32, dblp, dblp, dblp, dblp, dblp, dblp]
         L.whale_logpdf_noalloc.argtypes = [vp, C.c_int32, vp, vp]
         L.whale_download_ell.argtypes = [vp, C.c_int32, dblp, C.c_int64, i64p]
+        L.whale_backtrack.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, i32p, i32p]
         _lib = L
     return _lib
 

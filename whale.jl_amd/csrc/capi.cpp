@@ -162,6 +162,11 @@ struct whale_batch {
   // their first 8 bytes per element and, for FP32 runs, the FP64 recomputation
   // of families that left the FP32 range after the first align8(4 * n) bytes.
   DBuf ell, coef, E, ibar, fam_ll, acc, ticket, flag, fb;
+  // the families' split lists (per absolute clade: first split; per split:
+  // g1 | g2 << 16) and, built on the first backtracking call, each
+  // (family, node)'s segment and each segment's first item; the sampler's
+  // output and stack buffers
+  DBuf spl_off, spl_g, node_seg, seg_item, fam64, bt_counts, bt_nodes, bt_stack;
   // stage buffers of the launches that run them in global memory
   // (LaunchCfg::gws): the launches on the context's stream use the first
   // gws_main bytes, the FP64 evaluation of routed families (on its own
@@ -647,6 +652,13 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
   HIPCHK(upload(b->split_p, P.split_p, st));
   HIPCHK(upload(b->lvl, P.lvl, st));
   HIPCHK(upload(b->order, b->order_host, st));
+  {
+    std::vector<uint32_t> so(nc + 1), sg((size_t)split_off[nc]);
+    for (int64_t c = 0; c <= nc; ++c) so[c] = (uint32_t)split_off[c];
+    for (size_t k = 0; k < sg.size(); ++k) sg[k] = (uint32_t)g1[k] | ((uint32_t)g2[k] << 16);
+    HIPCHK(upload(b->spl_off, so, st));
+    HIPCHK(upload(b->spl_g, sg, st));
+  }
   b->n_rec = P.rec_ops.size();
   HIPCHK(b->ell.alloc(16 + 12 * std::max<uint64_t>(1, P.ell_cells)));
   HIPCHK(b->coef.alloc(16 + 12 * std::max<uint64_t>(1, b->n_rec)));
@@ -1473,6 +1485,82 @@ int whale_logpdf_noalloc(whale_batch* b, int32_t mode, double* d_family, double*
 int whale_logpdf_grad(whale_batch* b, int32_t mode, int32_t launch, double* out_family, double* out_total,
                       double* d_lambda, double* d_mu, double* d_q, double* d_eta) {
   return run(b, mode, launch, true, out_family, out_total, d_lambda, d_mu, d_q, d_eta);
+}
+
+int whale_backtrack(whale_batch* b, int32_t family, int32_t n_families, int32_t n_samples, uint64_t seed,
+                    int32_t cap, int32_t* counts, int32_t* nodes) {
+  if (!b || !counts || !nodes) return fail(WHALE_EINVAL, "NULL argument");
+  if (family < 0 || n_families < 1 || (uint64_t)family + n_families > b->n_fam)
+    return fail(WHALE_EINVAL, "families out of range");
+  if (n_samples < 1 || cap < 1) return fail(WHALE_EINVAL, "n_samples and cap must be positive");
+  if (b->last_mode < 0 || !b->last_store_all)
+    return fail(WHALE_ESTATE, "no stored ℓ: evaluate with WHALE_KEEP_ELL (logpdf!) or with the gradient first");
+  for (int32_t k = 0; k < n_families; ++k)
+    if (!std::isfinite(b->pin_fam[family + k]))
+      return fail(WHALE_EINVAL, "family " + std::to_string(family + k) + " has likelihood 0: nothing to backtrack");
+  whale_model* m = b->model;
+  whale_ctx* c = m->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  const PlanHost& P = b->plan;
+  if (!b->node_seg.p) {
+    std::vector<int32_t> ns((size_t)b->n_fam * m->h.n, -1);
+    for (uint32_t f = 0; f < b->n_fam; ++f)
+      for (uint32_t s = P.fam_seg0[f]; s < P.fam_seg0[f] + P.fam_nseg[f]; ++s)
+        ns[(size_t)f * m->h.n + P.seg[s].node] = (int32_t)s;
+    std::vector<uint32_t> si(P.seg.size(), 0);
+    for (size_t i = P.it_seg.size(); i-- > 0;) si[P.it_seg[i]] = (uint32_t)i;  // first item of each segment
+    HIPCHK(upload(b->node_seg, ns, st));
+    HIPCHK(upload(b->seg_item, si, st));
+  }
+  const int mode = b->last_mode;
+  std::vector<uint8_t> f64(b->n_fam, 0);
+  bool any64 = false;
+  if (mode == WHALE_FP32_SCALED) {
+    std::vector<uint32_t> flag(b->n_fam);
+    HIPCHK(hipMemcpy(flag.data(), b->flag.p, sizeof(uint32_t) * b->n_fam, hipMemcpyDeviceToHost));
+    for (uint32_t f = 0; f < b->n_fam; ++f) {
+      f64[f] = flag[f] || (b->routed[f] && b->last_launch == WHALE_LAUNCH_BATCHED);
+      any64 |= f64[f] != 0;
+    }
+    if (any64) HIPCHK(upload(b->fam64, f64, st));
+  }
+  BtDev D;
+  const BatchDev B = b->dev();
+  D.fam = B.fam;
+  D.seg = B.seg;
+  D.it_gid = B.it_gid;
+  D.seg_item = b->seg_item.as<uint32_t>();
+  D.node_seg = b->node_seg.as<int32_t>();
+  D.spl_off = b->spl_off.as<uint32_t>();
+  D.spl_g = b->spl_g.as<uint32_t>();
+  D.split_p = B.split_p;
+  D.clade_size = B.clade_size;
+  D.ell = b->ell.p;
+  D.ell64 = static_cast<const double*>(b->work_fallback().ell);
+  D.E = mode == WHALE_FP64_PLAIN ? nullptr : b->E.as<int32_t>();
+  D.fam64 = any64 ? b->fam64.as<uint8_t>() : nullptr;
+  D.ell_float = mode == WHALE_FP32_SCALED ? 1 : 0;
+  const ModelDev M = m->dev();
+  // lanes in chunks whose node and stack buffers stay within 256 MiB
+  const uint64_t per_lane = 16ull * (uint64_t)cap;
+  const uint64_t lanes_total = (uint64_t)n_families * n_samples;
+  const uint64_t fam_chunk = std::max<uint64_t>(1, (256ull << 20) / (per_lane * n_samples));
+  const uint64_t lanes_cap = std::min<uint64_t>(lanes_total, fam_chunk * n_samples);
+  HIPCHK(b->bt_counts.alloc(4 * lanes_cap));
+  HIPCHK(b->bt_nodes.alloc(per_lane * lanes_cap));
+  HIPCHK(b->bt_stack.alloc(per_lane * lanes_cap));
+  for (uint64_t f0 = 0; f0 < (uint64_t)n_families; f0 += fam_chunk) {
+    const uint32_t nf = (uint32_t)std::min<uint64_t>(fam_chunk, n_families - f0);
+    const uint64_t lanes = (uint64_t)nf * n_samples;
+    HIPCHK(launch_backtrack(st, D, M, (uint32_t)(family + f0), nf, (uint32_t)n_samples, seed, (uint32_t)cap,
+                            b->bt_counts.as<int32_t>(), b->bt_nodes.as<int32_t>(), b->bt_stack.as<int32_t>()));
+    HIPCHK(hipMemcpyAsync(counts + f0 * n_samples, b->bt_counts.p, 4 * lanes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nodes + f0 * n_samples * 4 * (uint64_t)cap, b->bt_nodes.p, per_lane * lanes,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return WHALE_OK;
 }
 
 int whale_download_ell(whale_batch* b, int32_t f, double* out, int64_t cap, int64_t* n_out) {

@@ -143,6 +143,30 @@ hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int n);
 hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n);
 
+// What the backtracking sampler reads: the stored ℓ (FP32 or FP64 values,
+// plus the FP64 region of families an FP32 call evaluated in FP64), the
+// clades' exponents, each (family, node)'s segment, each segment's first
+// item (its clades in increasing id), and every clade's split list.
+struct BtDev {
+  const FamDev* fam;
+  const SegDev* seg;
+  const uint32_t* it_gid;
+  const uint32_t* seg_item;   // absolute first item of each segment
+  const int32_t* node_seg;    // [family][node]: segment, or -1 (no compatible clade)
+  const uint32_t* spl_off;    // per absolute clade: its splits [spl_off[c], spl_off[c + 1])
+  const uint32_t* spl_g;      // per split: g1 | g2 << 16 (family-local clade ids)
+  const double* split_p;
+  const uint16_t* clade_size;
+  const void* ell;
+  const double* ell64;
+  const int32_t* E;           // null in the plain form
+  const uint8_t* fam64;       // per family: 1 if its ℓ is in ell64 (null: none)
+  int ell_float;              // the main region holds FP32 values
+};
+hipError_t launch_backtrack(hipStream_t st, const BtDev& D, const ModelDev& M, uint32_t fam_begin, uint32_t n_fams,
+                            uint32_t n_samples, uint64_t seed, uint32_t cap, int32_t* counts, int32_t* nodes,
+                            int32_t* stack);
+
 // The fixed per-GPU workspace of whale_logpdf_noalloc (a file-scope device
 // array in kernels.hip) and its device address on `device`, which must be the
 // current device; found once per device and kept.

@@ -2023,6 +2023,242 @@ hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// k_backtrack: stochastic backtracking of reconciled trees from the stored ℓ
+// (backtrack!, src/track.jl:188-414).  One lane per (family, sample): a depth
+// first walk over slice states (species node e, clade γ, slice row t) that
+// picks each event with probability term / ℓ_e(γ, t), the terms taken in the
+// reference's order: inside a branch "no event" (φ_t ℓ(γ, t-1)) then each
+// duplication (p ψ_t ℓ(γ1, t-1) ℓ(γ2, t-1)); at a speciation node the two
+// losses then both speciations of every split; at a WGD node non-retention
+// ((1 - q + 2qε_f) ℓ_f(γ)) then each retained split; at the root each split's
+// root duplication (ξ(1-η)/η) and both speciations (η(1-ε)/ξ²), then the two
+// losses.  Uniform draws come from a counter-based generator of (seed,
+// family, sample), one per inter- or intra-branch step in depth-first order.
+// Scaled forms: every term is formed relative to 2^E(γ) of the state's clade.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bt_uniform(uint64_t& st) {
+  st += 0x9E3779B97F4A7C15ull;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+struct BtCell {
+  double v;  // stored value (0 when the clade is not compatible with the branch)
+  int E;     // its clade's exponent (0 in the plain form)
+};
+
+struct BtCtx {
+  const BtDev& D;
+  const ModelDev& M;
+  FamDev F;
+  uint32_t f;
+  bool f64;  // this family's ℓ is in the FP64 region
+  __device__ BtCell cell(int e, int g, int t) const {
+    BtCell c{0.0, 0};
+    const int32_t s = D.node_seg[(size_t)f * M.n + e];
+    if (s < 0) return c;
+    const SegDev sg = D.seg[s];
+    if ((uint32_t)t > sg.R || t < 1) return c;
+    uint32_t col;
+    if (e == M.root) {
+      col = (uint32_t)g;  // the root's columns are all clades in order
+    } else {
+      // γ's column: the segment's clades are in increasing id
+      uint32_t lo = D.seg_item[s], hi = lo + sg.nc;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (D.it_gid[mid] < (uint32_t)g) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo == D.seg_item[s] + sg.nc || D.it_gid[lo] != (uint32_t)g) return c;
+      col = lo - D.seg_item[s];
+    }
+    const uint64_t idx = F.ell0 + sg.ell + (uint64_t)(t - 1) * sg.nc + col;
+    c.v = f64 ? D.ell64[idx] : (D.ell_float ? (double)((const float*)D.ell)[idx] : ((const double*)D.ell)[idx]);
+    if (D.E && c.v != 0.0) c.E = D.E[F.clade0 + g];
+    return c;
+  }
+  __device__ BtCell top(int e, int g) const { return cell(e, g, M.node[e].R); }
+};
+
+// term x·a·b relative to 2^Eg, a, b two cells
+__device__ __forceinline__ double bt_rel2(double x, const BtCell& a, const BtCell& b, int Eg) {
+  if (a.v == 0.0 || b.v == 0.0 || x == 0.0) return 0.0;
+  return ldexp(x * a.v * b.v, a.E + b.E - Eg);
+}
+__device__ __forceinline__ double bt_rel1(double x, const BtCell& a, int Eg) {
+  if (a.v == 0.0 || x == 0.0) return 0.0;
+  return ldexp(x * a.v, a.E - Eg);
+}
+
+struct BtState {
+  int e, g, t;
+};
+
+__global__ void k_backtrack(BtDev D, ModelDev M, uint32_t fam_begin, uint32_t n_fams, uint32_t n_samples, uint64_t seed,
+                            uint32_t cap, int32_t* counts, int32_t* nodes, int32_t* stack) {
+  const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (uint64_t)n_fams * n_samples) return;
+  const uint32_t fi = (uint32_t)(id / n_samples), smp = (uint32_t)(id % n_samples);
+  const uint32_t f = fam_begin + fi;
+  BtCtx X{D, M, D.fam[f], f, D.fam64 ? D.fam64[f] != 0 : false};
+  uint64_t st = seed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(f + 1)) ^ (0xD1B54A32D192ED03ull * (uint64_t)(smp + 1));
+  int32_t* out = nodes + id * (uint64_t)cap * 4;
+  int32_t* stk = stack + id * (uint64_t)cap * 4;
+  const double eta = M.par[3 * M.n + M.n_wgd];
+  uint32_t n = 0, sp = 0;
+  bool over = false;
+  auto add = [&](const BtState& s, int par) -> int {
+    if (n >= cap) { over = true; return -1; }
+    int32_t* o = out + 4 * (uint64_t)n;
+    o[0] = s.g; o[1] = s.e; o[2] = s.t; o[3] = par;
+    return (int)n++;
+  };
+  const int gam = (int)X.F.n_clades - 1;
+  BtState s{M.root, gam, 1};
+  int cur = add(s, -1);
+  for (;;) {
+    // the state s at node `cur`; a branch of the walk ends at a leaf or a loss
+    BtState nx[2];
+    int nn = 0;
+    bool leafend = false;
+    if (s.g < 0) {
+      leafend = true;
+    } else if (s.t == 1) {
+      const NodeDev& nd = M.node[s.e];
+      if (nd.kind == WHALE_NODE_LEAF) {
+        leafend = true;
+      } else {
+        const BtCell me = X.cell(s.e, s.g, 1);
+        const int Eg = me.E;
+        double r = bt_uniform(st) * me.v;
+        BtState last[2];
+        int nlast = 0;
+        auto term = [&](double x, BtState a, BtState b, int cnt) {
+          if (nn > 0 || !(x > 0.0)) return;
+          last[0] = a; last[1] = b; nlast = cnt;
+          r -= x;
+          if (r < 0.0) { nx[0] = a; nx[1] = b; nn = cnt; }
+        };
+        const uint32_t ca = X.F.clade0 + (uint32_t)s.g;
+        const uint32_t k0 = D.spl_off[ca], k1 = D.spl_off[ca + 1];
+        if (nd.kind == WHALE_NODE_ROOT) {
+          const double eps = eps_top(M, s.e), xi = 1.0 - (1.0 - eta) * eps;
+          const double cA = xi * (1.0 - eta) / eta, cB = eta * (1.0 - eps) / (xi * xi);
+          const int fc = nd.child0, gc = nd.child1;
+          const int tf = M.node[fc].R, tg = M.node[gc].R;
+          for (uint32_t k = k0; k < k1 && nn == 0; ++k) {
+            const int g1 = (int)(D.spl_g[k] & 0xffff), g2 = (int)(D.spl_g[k] >> 16);
+            const double p = D.split_p[k];
+            term(bt_rel2(p * cA, X.cell(s.e, g1, 1), X.cell(s.e, g2, 1), Eg), {s.e, g1, 1}, {s.e, g2, 1}, 2);
+            term(bt_rel2(p * cB, X.top(fc, g1), X.top(gc, g2), Eg), {fc, g1, tf}, {gc, g2, tg}, 2);
+            term(bt_rel2(p * cB, X.top(gc, g1), X.top(fc, g2), Eg), {gc, g1, tg}, {fc, g2, tf}, 2);
+          }
+          term(bt_rel1(eps_top(M, gc) * cB, X.top(fc, s.g), Eg), {fc, s.g, tf}, {gc, -1, 0}, 2);
+          term(bt_rel1(eps_top(M, fc) * cB, X.top(gc, s.g), Eg), {gc, s.g, tg}, {fc, -1, 0}, 2);
+        } else if (nd.kind == WHALE_NODE_WGD) {
+          const double q = M.par[3 * M.n + nd.wgd];
+          const int fc = nd.child0, tf = M.node[fc].R;
+          term(bt_rel1(1.0 - q + 2.0 * q * eps_top(M, fc), X.top(fc, s.g), Eg), {fc, s.g, tf}, {0, 0, 0}, 1);
+          for (uint32_t k = k0; k < k1 && nn == 0; ++k) {
+            const int g1 = (int)(D.spl_g[k] & 0xffff), g2 = (int)(D.spl_g[k] >> 16);
+            term(bt_rel2(q * D.split_p[k], X.top(fc, g1), X.top(fc, g2), Eg), {fc, g1, tf}, {fc, g2, tf}, 2);
+          }
+        } else {
+          const int fc = nd.child0, gc = nd.child1;
+          const int tf = M.node[fc].R, tg = M.node[gc].R;
+          term(bt_rel1(eps_top(M, gc), X.top(fc, s.g), Eg), {fc, s.g, tf}, {gc, -1, 0}, 2);
+          term(bt_rel1(eps_top(M, fc), X.top(gc, s.g), Eg), {gc, s.g, tg}, {fc, -1, 0}, 2);
+          for (uint32_t k = k0; k < k1 && nn == 0; ++k) {
+            const int g1 = (int)(D.spl_g[k] & 0xffff), g2 = (int)(D.spl_g[k] >> 16);
+            const double p = D.split_p[k];
+            term(bt_rel2(p, X.top(fc, g1), X.top(gc, g2), Eg), {fc, g1, tf}, {gc, g2, tg}, 2);
+            term(bt_rel2(p, X.top(gc, g1), X.top(fc, g2), Eg), {gc, g1, tg}, {fc, g2, tf}, 2);
+          }
+        }
+        if (nn == 0) {  // rounding left r >= 0: the last non-zero term
+          nn = nlast;
+          nx[0] = last[0];
+          nx[1] = last[1];
+        }
+        if (nn == 0) leafend = true;
+      }
+    } else if (D.clade_size[X.F.clade0 + (uint32_t)s.g] == 1) {
+      s.t = 1;  // a one-gene clade does not duplicate along the branch
+      continue;
+    } else {
+      const NodeDev& nd = M.node[s.e];
+      const BtCell me = X.cell(s.e, s.g, s.t);
+      const int Eg = me.E;
+      double r = bt_uniform(st) * me.v;
+      const double stay = bt_rel1(M.phi[nd.tab + s.t - 1], X.cell(s.e, s.g, s.t - 1), Eg);
+      r -= stay;
+      if (r < 0.0) {
+        s.t -= 1;
+        continue;
+      }
+      const double ps = M.psi[nd.tab + s.t - 1];
+      const uint32_t ca = X.F.clade0 + (uint32_t)s.g;
+      bool any = false;
+      for (uint32_t k = D.spl_off[ca]; k < D.spl_off[ca + 1]; ++k) {
+        const int g1 = (int)(D.spl_g[k] & 0xffff), g2 = (int)(D.spl_g[k] >> 16);
+        const double x = bt_rel2(D.split_p[k] * ps, X.cell(s.e, g1, s.t - 1), X.cell(s.e, g2, s.t - 1), Eg);
+        if (x > 0.0) {
+          nx[0] = {s.e, g1, s.t - 1};
+          nx[1] = {s.e, g2, s.t - 1};
+          any = true;
+          r -= x;
+          if (r < 0.0) break;
+        }
+      }
+      if (any) {
+        nn = 2;  // the split where r fell below 0, or (rounding) the last non-zero one
+      } else {
+        s.t -= 1;  // only "no event" has weight
+        continue;
+      }
+    }
+    if (!leafend) {
+      // next states in the reference's order: the first is walked first
+      for (int j = nn - 1; j >= 0; --j) {
+        if (sp >= cap) { over = true; break; }
+        int32_t* q = stk + 4 * (uint64_t)sp++;
+        q[0] = nx[j].e; q[1] = nx[j].g; q[2] = nx[j].t; q[3] = cur;
+      }
+    }
+    if (over || sp == 0) break;
+    // the next pending state: a new node when its (clade, branch) differs
+    // from the node it came from ((b::BackTracker)(newstate), src/track.jl:172-180)
+    const int32_t* q = stk + 4 * (uint64_t)(--sp);
+    const BtState ns{q[0], q[1], q[2]};
+    const int from = q[3];
+    const int32_t* fn = out + 4 * (uint64_t)from;
+    if (ns.g != fn[0] || ns.e != fn[1]) {
+      cur = add(ns, from);
+      if (cur < 0) break;
+    } else {
+      cur = from;
+    }
+    s = ns;
+  }
+  counts[id] = over ? -1 : (int32_t)n;
+}
+
+hipError_t launch_backtrack(hipStream_t st, const BtDev& D, const ModelDev& M, uint32_t fam_begin, uint32_t n_fams,
+                            uint32_t n_samples, uint64_t seed, uint32_t cap, int32_t* counts, int32_t* nodes,
+                            int32_t* stack) {
+  const uint64_t lanes = (uint64_t)n_fams * n_samples;
+  if (lanes == 0) return hipSuccess;
+  const uint32_t grid = (uint32_t)((lanes + 63) / 64);
+  hipLaunchKernelGGL(k_backtrack, dim3(grid), dim3(64), 0, st, D, M, fam_begin, n_fams, n_samples, seed, cap, counts,
+                     nodes, stack);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // fixed workspace of the evaluation that allocates nothing
 // ---------------------------------------------------------------------------
 // Every GPU the library runs on holds one copy of this array until the
