@@ -548,11 +548,627 @@ struct FwdRoot {
 // ---------------------------------------------------------------------------
 constexpr int kMaxFPW = 8;  // families (waves) per workgroup, at most
 
+// The forward DP of family f by one wave: its stages (row buffers in the
+// wave's region wb), then its root.  ℓ is written through `ell` (the
+// family's cells, relative indices) and record coefficients through `coef`
+// (absolute record indices).
+template <typename Real, bool Scaled, bool StoreAll, bool TabL, bool Gws>
+__device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M, const WorkDev<Real>& W,
+                                           const LdsCfg& L, uint32_t f, unsigned char* const wb, Real* sTab,
+                                           const NodeC<Real>& NC, uint32_t lane, Real* ell, Real* coef) {
+  constexpr int RC = 2;  // records per lane held in registers (128 per stage)
+  const FamDev F = B.fam[f];
+  int32_t* E = W.E + F.clade0;
+  if (Scaled)
+    for (uint32_t c = lane; c < F.n_clades; c += kWG) E[c] = kUndef;
+  // ---------------- non-root stages
+  for (uint32_t s = 0; s < F.n_stages; ++s) {
+    const StageDev S = B.stage[F.stage0 + s];
+    const uint32_t n = S.n_items, i0 = S.item0;
+    const uint32_t rb = S.rec0, nrec = S.n_recs;
+    const FwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
+    Real* const bA = T.bA;
+    Real* const bB = T.bB;
+    Real* const bF = T.bF;
+    Real* const sAcc = T.sAcc;
+    Real* const sCoef = T.sCoef;
+    Real* const sP = T.sP;
+    uint32_t* const sOps = T.sOps;
+    uint16_t* const sOwn = T.sOwn;
+    uint32_t* const sRO = T.sRO;
+    uint16_t* const sSegL = T.sSegL;
+    int32_t* const sEi = T.sEi;
+    uint8_t* const sFlag = T.sFlag;
+    wave_sync_mem();  // ℓ and exponents of the stages below
+    SegTab ST;
+    ST.load(B, M, S.seg0, S.n_segs, lane);
+    for (uint32_t i = lane; i < n; i += kWG) {
+      const uint32_t ia = i0 + i;
+      const int32_t cf = B.it_cf[ia], cg = B.it_cg[ia];
+      sSegL[i] = (uint16_t)(B.it_seg[ia] - S.seg0);
+      bF[i] = cf >= 0 ? ell[cf] : Real(0);
+      bB[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops (bB is free until row 1)
+      sAcc[i] = 0;
+      sRO[i] = B.it_rec[ia] - rb;
+      if (Scaled) {
+        const int e = E[B.it_gid[ia]];
+        sEi[i] = e;
+        sFlag[i] = e == kUndef ? (kPending | kYoung) : 0;
+      }
+    }
+    if (lane == 0) sRO[n] = nrec;
+#pragma unroll 4
+    for (uint32_t k = lane; k < nrec; k += kWG) {
+      sOps[k] = B.rec_ops[rb + k];
+      sOwn[k] = B.rec_own[rb + k];
+      const Real p = (Real)B.rec_p[rb + k];
+      sCoef[k] = p;  // the scaled forms rescale it after row 0
+      sP[k] = p;
+    }
+    wsync<Gws>();
+    // Row 0 of a stage without WGD segments: the split terms are summed
+    // record-parallel first (into sAcc, which is zero until the row loop),
+    // so the item pass below only adds the direct terms.  A clade still
+    // without an exponent ("open") takes the largest exponent of its terms:
+    // the first record pass raises sEi[own] to kOpen + that exponent, above
+    // every real exponent; the item pass normalises and fixes it.
+    constexpr int kOpen = 1 << 28;
+    const bool fast0 = ST.inreg && !__any(lane < S.n_segs && ST.mine.kind == WHALE_NODE_WGD);
+    if (fast0) {
+      if (Scaled) {
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          const uint32_t own = sOwn[k];
+          const int Eo = sEi[own];
+          if (Eo != kUndef && Eo < kOpen / 2) continue;
+          const uint32_t ops = sOps[k];
+          const uint32_t a = ops & 0xffff, b = ops >> 16;
+          const Real pk = sCoef[k];
+          const Real fa = bF[a], fb = bF[b], ga = bB[a], gb = bB[b];
+          const int ea = sEi[a], eb = sEi[b];
+          int emax = kUndef;
+          Real m;
+          int ee, em;
+          if (fa != Real(0) && gb != Real(0)) { xprod(pk, fa, ea, gb, eb, &m, &ee); (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+          if (ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ee); (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+          if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
+        }
+        wsync<Gws>();
+      }
+      for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the run sums use DPP
+        const uint32_t k = base + lane;
+        uint32_t own = kNone;
+        Real t = 0;
+        if (k < nrec) {
+          own = sOwn[k];
+          const uint32_t ops = sOps[k];
+          const uint32_t a = ops & 0xffff, b = ops >> 16;
+          const Real pk = sCoef[k];
+          const Real fa = bF[a], fb = bF[b], ga = bB[a], gb = bB[b];
+          if (!Scaled) {
+            t = pk * (fa * gb + ga * fb);
+          } else {
+            const int Eo = sEi[own], ea = sEi[a], eb = sEi[b];
+            if (Eo >= kOpen / 2) {
+              const int K = Eo - kOpen;
+              Real m;
+              int ee;
+              if (fa != Real(0) && gb != Real(0)) { xprod(pk, fa, ea, gb, eb, &m, &ee); t += ldexp_r(m, ee - K); }
+              if (ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ee); t += ldexp_r(m, ee - K); }
+            } else if (Eo != kUndef && ea != kUndef && eb != kUndef) {
+              // fixed exponent: the term formed in FP64 (wide enough for any
+              // product of the stored values), rescaled to it
+              t = (Real)ldexp((double)pk * ((double)fa * gb + (double)ga * fb), ea + eb - Eo);
+            }
+          }
+        }
+        bool last;
+        t = row_seg_sum(t, own, &last);
+        if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
+      }
+      wsync<Gws>();
+    }
+    // row 0: terms from the child branches (whale!/whalewgd!, src/core.jl:95-128)
+    auto row0 = [&](const SegInfo& sg, uint32_t i) {
+      const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
+      Real v = 0;
+      if (fast0 && sg.kind != WHALE_NODE_LEAF) {
+        // speciation: the split terms are in sAcc
+        const Real sp = sAcc[i];
+        sAcc[i] = 0;
+        const Real ef = NC.etop[sg.c0], eg = NC.etop[sg.c1];
+        if (!Scaled) {
+          v = sp + (bF[i] * eg + bB[i] * ef);
+        } else {
+          const int Eg = sEi[i];
+          if (Eg >= kOpen / 2) {
+            int e2;
+            v = ldexp_r(frexp_r(sp, &e2), -kBirth);
+            sEi[i] = Eg - kOpen + e2 + kBirth;
+            sFlag[i] = kYoung;
+          } else if (Eg != kUndef) {
+            v = (bF[i] * eg + bB[i] * ef) + sp;
+          }
+        }
+      } else if (sg.kind == WHALE_NODE_LEAF) {
+        if (r1 == r0) {  // a clade of one gene (every larger clade has a split)
+          const Real lp = NC.leafP[sg.node];
+          if (Scaled) {
+            const int Eg = sEi[i];
+            if (Eg == kUndef) {
+              int ex;
+              v = frexp_r(lp, &ex);
+              sEi[i] = ex;
+              sFlag[i] = kYoung;
+            } else {
+              v = ldexp_r(lp, -Eg);
+            }
+          } else {
+            v = lp;
+          }
+        }
+      } else if (!Scaled) {
+        if (sg.kind == WHALE_NODE_WGD) {
+          const Real qq = NC.q[sg.wgd];
+          const Real ef = NC.etop[sg.c0];
+          Real r = 0;
+          for (uint32_t k = r0; k < r1; ++k) {
+            const uint32_t ops = sOps[k];
+            r += sCoef[k] * bF[ops & 0xffff] * bF[ops >> 16];
+          }
+          v = r * qq + ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]);
+        } else {
+          const Real ef = NC.etop[sg.c0], eg = NC.etop[sg.c1];
+          Real sp = 0;
+          for (uint32_t k = r0; k < r1; ++k) {
+            const uint32_t ops = sOps[k];
+            const uint32_t a = ops & 0xffff, b = ops >> 16;
+            sp += sCoef[k] * (bF[a] * bB[b] + bB[a] * bF[b]);
+          }
+          v = sp + (bF[i] * eg + bB[i] * ef);
+        }
+      } else {
+        // scaled: explicit exponents (the operands come from other segments)
+        const bool wgd = sg.kind == WHALE_NODE_WGD;
+        const Real qq = wgd ? NC.q[sg.wgd] : Real(1);
+        const Real ef = NC.etop[sg.c0];
+        const Real eg = wgd ? Real(0) : NC.etop[sg.c1];
+        const int Eg = sEi[i];
+        if (Eg != kUndef) {
+          // exponent fixed: the split terms, rescaled to it, summed in FP64
+          // (wide enough for any product of the stored values)
+          double sacc = 0.0;
+          for (uint32_t k = r0; k < r1; ++k) {
+            const uint32_t ops = sOps[k];
+            const uint32_t a = ops & 0xffff, b = ops >> 16;
+            const int ea = sEi[a], eb = sEi[b];
+            if (ea == kUndef || eb == kUndef) continue;  // parts without values yet
+            const double fa = bF[a], fb = bF[b];
+            const double t = wgd ? (double)qq * fa * fb : fa * (double)bB[b] + (double)bB[a] * fb;
+            sacc += ldexp((double)sCoef[k] * t, ea + eb - Eg);
+          }
+          const Real base = wgd ? ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]) : (bF[i] * eg + bB[i] * ef);
+          v = base + (Real)sacc;
+        } else {
+          // first values of this clade: explicit exponents, then its own
+          XSum<Real> xs;
+          for (uint32_t k = r0; k < r1; ++k) {
+            const uint32_t ops = sOps[k];
+            const uint32_t a = ops & 0xffff, b = ops >> 16;
+            const Real pk = sCoef[k];
+            const Real fa = bF[a], fb = bF[b];
+            const Real ga = wgd ? fa : bB[a], gb = wgd ? fb : bB[b];
+            const int ea = sEi[a], eb = sEi[b];
+            Real m;
+            int ex;
+            if (fa != Real(0) && gb != Real(0)) { xprod(pk * qq, fa, ea, gb, eb, &m, &ex); xs.add(m, ex); }
+            if (!wgd && ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ex); xs.add(m, ex); }
+          }
+          if (xs.k != kUndef) {
+            int ex;
+            v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
+            sEi[i] = xs.k + ex + kBirth;
+            sFlag[i] = kYoung;
+          }
+        }
+      }
+      bA[i] = v;
+      if (StoreAll || sg.R == 1) ell[sg.ell + (i - sg.item0)] = v;
+    };
+    // row 0 of every item; the first kCache per lane also fill the register cache
+    ItemRow cache[kCache];
+#pragma unroll
+    for (int c = 0; c < kCache; ++c) {
+      if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
+        const uint32_t i = lane + c * kWG;
+        const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+        if (i < n) {
+          cache[c] = item_row(sg, i);
+          row0(sg, i);
+        }
+      }
+    }
+    for (uint32_t base = kCache * kWG; base < n; base += kWG) {
+      const uint32_t i = base + lane;
+      const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+      if (i < n) row0(sg, i);
+    }
+    wsync<Gws>();
+    // scaled forms: coefficients for the exponents now fixed.  A record whose
+    // clade is still open keeps p (the open clade sums explicit exponents).
+    auto refresh = [&]() {
+      for (uint32_t k = lane; k < nrec; k += kWG) {
+        const uint32_t ops = sOps[k];
+        const int Eg = sEi[sOwn[k]];
+        const Real p = sP[k];
+        const Real c = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
+        sCoef[k] = Eg == kUndef ? p : c;
+        if (StoreAll) coef[rb + k] = c;
+      }
+    };
+    // items still without values (open clades with splits): they take the
+    // explicit-exponent path until their first value
+    auto count_open = [&]() -> uint32_t {
+      uint32_t cnt = 0;
+      for (uint32_t base = 0; base < n; base += kWG) {
+        const uint32_t i = base + lane;
+        const bool open = i < n && (sFlag[i] & kPending) && sRO[i + 1] > sRO[i];
+        cnt += (uint32_t)__popcll(__ballot(open));
+      }
+      return cnt;
+    };
+    uint32_t n_open = 0;
+    if (Scaled) {
+      refresh();
+      wsync<Gws>();
+      n_open = count_open();
+    }
+    // the first RC records per lane in registers: operands, owner, coefficient
+    // (row-invariant until the coefficients are refreshed)
+    uint32_t rops[RC], rown[RC];
+    Real rcoef[RC];
+    RunMask rm[RC];
+    auto load_recs = [&]() {
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const uint32_t k = lane + c * kWG;
+        const bool in = k < nrec;
+        rops[c] = in ? sOps[k] : 0u;
+        rown[c] = in ? (uint32_t)sOwn[k] : kNone;
+        rcoef[c] = in ? sCoef[k] : Real(0);
+        rm[c] = run_mask(rown[c]);
+      }
+    };
+    load_recs();
+    // rows 1..R-1: within_branch! (src/core.jl:83-92)
+    Real* cur = bA;
+    Real* nxt = bB;
+    for (uint32_t r = 1; r < S.r_max; ++r) {
+      bool dirty = false;
+      // split sums of every item, one record per lane (lane-balanced however
+      // the records are spread over the items); the cached records' operand
+      // reads are all issued before any is used
+      {
+        Real va[RC], vb[RC];
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+          if ((uint32_t)c * kWG < nrec) {
+            va[c] = cur[rops[c] & 0xffff];
+            vb[c] = cur[rops[c] >> 16];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+          if ((uint32_t)c * kWG < nrec) {  // uniform: the row sums use DPP
+            const Real t = run_sum(rcoef[c] * va[c] * vb[c], rm[c]);
+            if (rm[c].ends() && t != Real(0)) lds_add(&sAcc[rown[c]], t);
+          }
+        }
+      }
+      for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: the row sums use DPP
+        const uint32_t k = base + lane;
+        uint32_t own = kNone;
+        Real t = 0;
+        if (k < nrec) {
+          const uint32_t ops = sOps[k];
+          own = sOwn[k];
+          t = sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
+        }
+        bool last;
+        t = row_seg_sum(t, own, &last);
+        if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
+      }
+      wsync<Gws>();
+      // a young clade past 2^kBump is renormalised in place (its only values are
+      // this branch's rows); its exponent is raised after the row, as other items
+      // of this row still read the clade's previous row at the old scale
+      auto bump_check = [&](const ItemRow& it, uint32_t i, Real& v, uint8_t fl) {
+        if (Scaled && sizeof(Real) == sizeof(float) && (fl & kYoung) &&
+            fabsf((float)v) > (float)(1ull << kBump) && isfinite((float)v)) {
+          int ex;
+          (void)frexp_r(v, &ex);
+          const int sh = ex + kBirth;
+          v = ldexp_r(v, -sh);
+          if (StoreAll)
+            for (uint32_t q = 0; q < r; ++q) {
+              Real* p = ell + it.cell + (size_t)q * it.stride;
+              *p = ldexp_r(*p, -sh);
+            }
+          bF[i] = (Real)sh;  // bF is free after row 0
+          sFlag[i] = fl | kShifted;
+          dirty = true;
+        }
+      };
+      auto update = [&](const ItemRow& it, uint32_t i) {
+        Real v = cur[i];
+        if (r < it.R) {
+          Real ph, ps;
+          slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
+          const uint8_t fl = Scaled ? sFlag[i] : 0;
+          const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
+          if (!Scaled || !(fl & kPending) || r1 == r0) {
+            v = ph * v + ps * sAcc[i];
+            bump_check(it, i, v, fl);
+          } else {
+            // clade still open: sum with explicit exponents (coefficients hold p)
+            XSum<Real> xs;
+            for (uint32_t k = r0; k < r1; ++k) {
+              const uint32_t ops = sOps[k];
+              const uint32_t a = ops & 0xffff, b = ops >> 16;
+              const Real va = cur[a], vb = cur[b];
+              if (va == Real(0) || vb == Real(0)) continue;
+              Real m;
+              int ex;
+              xprod(sCoef[k], va, sEi[a], vb, sEi[b], &m, &ex);
+              xs.add(m, ex);
+            }
+            if (xs.k != kUndef) {
+              int ex;
+              v = ldexp_r(frexp_r(ps * xs.s, &ex), -kBirth);
+              sEi[i] = xs.k + ex + kBirth;  // readers of row r-1 see 0 here and skip it
+              sFlag[i] = kYoung;
+              dirty = true;
+            }
+          }
+          if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
+        }
+        nxt[i] = v;
+        sAcc[i] = 0;
+      };
+      if (!Scaled || n_open == 0) {
+        // no clade of the stage is still open: every cached item takes the
+        // plain update; all slots' LDS reads are issued before any is used
+        Real v0[kCache], ac[kCache], ph[kCache], ps[kCache];
+        uint8_t fl[kCache];
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          if ((uint32_t)c * kWG < n) {
+            const uint32_t i = lane + c * kWG;
+            const uint32_t ii = i < n ? i : 0u;
+            const bool act = i < n && r < cache[c].R;
+            slice<Real, TabL>(M, sTab, act ? cache[c].tab + r : 0u, ph[c], ps[c]);
+            v0[c] = cur[ii];
+            ac[c] = sAcc[ii];
+            fl[c] = (Scaled && sizeof(Real) == sizeof(float)) ? sFlag[ii] : 0;
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          if ((uint32_t)c * kWG < n) {
+            const uint32_t i = lane + c * kWG;
+            if (i < n) {
+              const ItemRow& it = cache[c];
+              Real v = v0[c];
+              if (r < it.R) {
+                v = ph[c] * v + ps[c] * ac[c];
+                bump_check(it, i, v, fl[c]);
+                if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
+              }
+              nxt[i] = v;
+              sAcc[i] = 0;
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < kCache; ++c) {
+          const uint32_t i = lane + c * kWG;
+          if (i < n) update(cache[c], i);
+        }
+      }
+      for (uint32_t base = kCache * kWG; base < n; base += kWG) {  // uniform: lane shuffles
+        const uint32_t i = base + lane;
+        const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
+        if (i < n) update(it, i);
+      }
+      wsync<Gws>();
+      if (Scaled && __any(dirty)) {
+        for (uint32_t i = lane; i < n; i += kWG) {
+          if (sFlag[i] & kShifted) {
+            sEi[i] += (int)bF[i];
+            sFlag[i] &= (uint8_t)~kShifted;
+          }
+        }
+        wsync<Gws>();
+        refresh();
+        wsync<Gws>();
+        n_open = count_open();
+        load_recs();
+      }
+      Real* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+    // exponents of this stage's clades, for the stages above and ℓ download
+    if (Scaled)
+      for (uint32_t i = lane; i < n; i += kWG) E[B.it_gid[i0 + i]] = sEi[i];
+  }
+  // ---------------- root: all clades, by size level (whaleroot!, src/core.jl:139-158)
+  {
+    wave_sync_mem();
+    const uint32_t n = F.n_clades;
+    const uint32_t i0 = F.root_item0;
+    const SegDev sg = B.seg[F.root_seg];
+    const NodeDev& nd = M.node[M.root];
+    const double eta = M.par[3 * M.n + M.n_wgd];
+    const double er = eps_top(M, M.root);
+    const double xi = 1.0 - (1.0 - eta) * er;
+    const Real cA = (Real)((1.0 - eta) * xi / eta);
+    const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
+    const Real ef = NC.etop[nd.child0], eg = NC.etop[nd.child1];
+    const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
+    // records staged in LDS when they fit the wave's region
+    const bool inl = fwd_root_bytes(sizeof(Real), Scaled, n, nrec) <= L.wave_bytes;
+    const FwdRoot<Real> T(wb, n, inl ? nrec : 0);
+    Real* const bA = T.bA;
+    Real* const bF = T.bF;
+    Real* const bB = T.bB;
+    Real* const sCoef = T.sCoef;
+    uint32_t* const sOps = T.sOps;
+    uint16_t* const sOwn = T.sOwn;
+    uint32_t* const sRO = T.sRO;
+    int32_t* const sEi = T.sEi;
+    for (uint32_t i = lane; i < n; i += kWG) {
+      const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
+      bF[i] = cf >= 0 ? ell[cf] : Real(0);
+      bB[i] = cg >= 0 ? ell[cg] : Real(0);
+      sRO[i] = B.it_rec[i0 + i] - rb;
+      if (Scaled) sEi[i] = E[i];  // root items are the clades in order
+    }
+    if (lane == 0) sRO[n] = nrec;
+    if (inl) {
+#pragma unroll 4
+      for (uint32_t k = lane; k < nrec; k += kWG) {
+        sOps[k] = B.rec_ops[rb + k];
+        sOwn[k] = B.rec_own[rb + k];
+        sCoef[k] = (Real)B.rec_p[rb + k];
+      }
+    }
+    const LevelBounds LB(B, F, lane);
+    // bA accumulates each clade's root value: it starts at the direct term
+    // cB·(F(γ)ε_g + G(γ)ε_f) (zero for a clade without a value below the
+    // root), then the level passes add the split terms of its records.
+    for (uint32_t i = lane; i < n; i += kWG) bA[i] = cB * (bF[i] * eg + bB[i] * ef);
+    wsync<Gws>();
+    // record k of a level: operands, p, owner
+    auto rrec = [&](uint32_t k, uint32_t& ops, Real& p, uint32_t& own) {
+      if (inl) { ops = sOps[k]; p = sCoef[k]; own = sOwn[k]; }
+      else { ops = B.rec_ops[rb + k]; p = (Real)B.rec_p[rb + k]; own = B.rec_own[rb + k]; }
+    };
+    // Scaled forms: a clade still without an exponent at its level ("open")
+    // takes the largest exponent of its split terms.  The first record pass
+    // raises sEi[own] to kOpen + that exponent (above every real exponent,
+    // so the open owners stay recognisable); the second adds each term
+    // rescaled to its owner's exponent; the item pass normalises the open
+    // clades' sums and fixes their exponents.
+    constexpr int kOpen = 1 << 28;
+    for (uint32_t Lv = 0; Lv < F.n_lvl; ++Lv) {
+      const uint32_t a0 = LB.at(B, F, Lv), a1 = LB.at(B, F, Lv + 1);
+      const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sRO[a0]);
+      const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sRO[a1]);
+      if (k0 == k1) continue;  // clades without splits keep their direct term
+      if (Scaled) {
+        for (uint32_t k = k0 + lane; k < k1; k += kWG) {
+          uint32_t ops, own;
+          Real pk;
+          rrec(k, ops, pk, own);
+          const int Eo = sEi[own];
+          if (Eo != kUndef && Eo < kOpen / 2) continue;  // exponent fixed below the root
+          const uint32_t x = ops & 0xffff, y = ops >> 16;
+          const int ex_ = sEi[x], ey = sEi[y];
+          int emax = kUndef;
+          Real m;
+          int ee;
+          if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+          if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+          if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
+          if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
+        }
+        wsync<Gws>();
+      }
+      for (uint32_t base = k0; base < k1; base += kWG) {  // uniform: the run sums use DPP
+        const uint32_t k = base + lane;
+        uint32_t own = kNone;
+        Real t = 0;
+        if (k < k1) {
+          uint32_t ops;
+          Real pk;
+          rrec(k, ops, pk, own);
+          const uint32_t x = ops & 0xffff, y = ops >> 16;
+          if (!Scaled) {
+            t = pk * (cA * bA[x] * bA[y] + cB * (bF[x] * bB[y] + bB[x] * bF[y]));
+          } else {
+            const int Eo = sEi[own], ex_ = sEi[x], ey = sEi[y];
+            if (Eo >= kOpen / 2) {
+              // open owner: each product as (mantissa, exponent) against its largest
+              const int K = Eo - kOpen;
+              Real m;
+              int ee;
+              if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
+              if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
+              if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
+            } else if (Eo != kUndef && ex_ != kUndef && ey != kUndef) {
+              // fixed exponent: the term rescaled to it, formed in FP64 (wide
+              // enough for any product of the stored values)
+              const double tt = (double)cA * bA[x] * bA[y] + (double)cB * ((double)bF[x] * bB[y] + (double)bB[x] * bF[y]);
+              t = (Real)ldexp((double)pk * tt, ex_ + ey - Eo);
+            }
+          }
+        }
+        bool last;
+        t = row_seg_sum(t, own, &last);
+        if (last && own != kNone && t != Real(0)) lds_add(&bA[own], t);
+      }
+      wsync<Gws>();
+      if (Scaled) {
+        for (uint32_t i = a0 + lane; i < a1; i += kWG) {
+          const int Eo = sEi[i];
+          if (Eo >= kOpen / 2) {
+            int e2;
+            const Real mm = frexp_r(bA[i], &e2);
+            bA[i] = ldexp_r(mm, -kBirth);
+            sEi[i] = Eo - kOpen + e2 + kBirth;
+          }
+        }
+        wsync<Gws>();
+      }
+    }
+    for (uint32_t i = lane; i < n; i += kWG) {
+      ell[sg.ell + i] = bA[i];
+      if (Scaled) E[i] = sEi[i];
+    }
+    if (lane == 0) {
+      const Real Lr = bA[n - 1];
+      double ll;
+      if (Lr > Real(0) && isfinite((double)Lr)) {
+        ll = log((double)Lr);
+        if (Scaled) ll += (double)sEi[n - 1] * 0.69314718055994530942;
+      } else {
+        ll = -INFINITY;
+        if (Scaled && sizeof(Real) == sizeof(float)) {
+          // overflow, or everything underflowed: redo this family in FP64 (a true zero stays -Inf there)
+          W.flag[f] = 1u;
+          W.fb[1 + atomicAdd(W.fb, 1u)] = f;
+        }
+      }
+      W.fam_ll[f] = ll;
+    }
+    if (Scaled && StoreAll) {
+      // final coefficients of the root records, for the reverse sweep
+      for (uint32_t k = lane; k < nrec; k += kWG) {
+        const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
+        const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
+        const int Eg = sEi[inl ? sOwn[k] : B.rec_own[rb + k]];
+        coef[rb + k] = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
+      }
+    }
+  }
+}
+
+
 template <typename Real, bool Scaled, bool StoreAll, bool TabL, bool Gws>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                        const uint32_t* fam_count) {
-  constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x % kWG;
@@ -585,612 +1201,8 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
       f = __builtin_amdgcn_readfirstlane(B.order[k]);  // wave-uniform: family data in scalar registers
     }
     first = false;
-    const FamDev F = B.fam[f];
-    Real* ell = W.ell + F.ell0;
-    int32_t* E = W.E + F.clade0;
-    if (Scaled)
-      for (uint32_t c = lane; c < F.n_clades; c += kWG) E[c] = kUndef;
-    // ---------------- non-root stages
-    for (uint32_t s = 0; s < F.n_stages; ++s) {
-      const StageDev S = B.stage[F.stage0 + s];
-      const uint32_t n = S.n_items, i0 = S.item0;
-      const uint32_t rb = S.rec0, nrec = S.n_recs;
-      const FwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
-      Real* const bA = T.bA;
-      Real* const bB = T.bB;
-      Real* const bF = T.bF;
-      Real* const sAcc = T.sAcc;
-      Real* const sCoef = T.sCoef;
-      Real* const sP = T.sP;
-      uint32_t* const sOps = T.sOps;
-      uint16_t* const sOwn = T.sOwn;
-      uint32_t* const sRO = T.sRO;
-      uint16_t* const sSegL = T.sSegL;
-      int32_t* const sEi = T.sEi;
-      uint8_t* const sFlag = T.sFlag;
-      wave_sync_mem();  // ℓ and exponents of the stages below
-      SegTab ST;
-      ST.load(B, M, S.seg0, S.n_segs, lane);
-      for (uint32_t i = lane; i < n; i += kWG) {
-        const uint32_t ia = i0 + i;
-        const int32_t cf = B.it_cf[ia], cg = B.it_cg[ia];
-        sSegL[i] = (uint16_t)(B.it_seg[ia] - S.seg0);
-        bF[i] = cf >= 0 ? ell[cf] : Real(0);
-        bB[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops (bB is free until row 1)
-        sAcc[i] = 0;
-        sRO[i] = B.it_rec[ia] - rb;
-        if (Scaled) {
-          const int e = E[B.it_gid[ia]];
-          sEi[i] = e;
-          sFlag[i] = e == kUndef ? (kPending | kYoung) : 0;
-        }
-      }
-      if (lane == 0) sRO[n] = nrec;
-#pragma unroll 4
-      for (uint32_t k = lane; k < nrec; k += kWG) {
-        sOps[k] = B.rec_ops[rb + k];
-        sOwn[k] = B.rec_own[rb + k];
-        const Real p = (Real)B.rec_p[rb + k];
-        sCoef[k] = p;  // the scaled forms rescale it after row 0
-        sP[k] = p;
-      }
-      wsync<Gws>();
-      // Row 0 of a stage without WGD segments: the split terms are summed
-      // record-parallel first (into sAcc, which is zero until the row loop),
-      // so the item pass below only adds the direct terms.  A clade still
-      // without an exponent ("open") takes the largest exponent of its terms:
-      // the first record pass raises sEi[own] to kOpen + that exponent, above
-      // every real exponent; the item pass normalises and fixes it.
-      constexpr int kOpen = 1 << 28;
-      const bool fast0 = ST.inreg && !__any(lane < S.n_segs && ST.mine.kind == WHALE_NODE_WGD);
-      if (fast0) {
-        if (Scaled) {
-          for (uint32_t k = lane; k < nrec; k += kWG) {
-            const uint32_t own = sOwn[k];
-            const int Eo = sEi[own];
-            if (Eo != kUndef && Eo < kOpen / 2) continue;
-            const uint32_t ops = sOps[k];
-            const uint32_t a = ops & 0xffff, b = ops >> 16;
-            const Real pk = sCoef[k];
-            const Real fa = bF[a], fb = bF[b], ga = bB[a], gb = bB[b];
-            const int ea = sEi[a], eb = sEi[b];
-            int emax = kUndef;
-            Real m;
-            int ee, em;
-            if (fa != Real(0) && gb != Real(0)) { xprod(pk, fa, ea, gb, eb, &m, &ee); (void)frexp_r(m, &em); emax = max(emax, ee + em); }
-            if (ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ee); (void)frexp_r(m, &em); emax = max(emax, ee + em); }
-            if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
-          }
-          wsync<Gws>();
-        }
-        for (uint32_t base = 0; base < nrec; base += kWG) {  // uniform: the run sums use DPP
-          const uint32_t k = base + lane;
-          uint32_t own = kNone;
-          Real t = 0;
-          if (k < nrec) {
-            own = sOwn[k];
-            const uint32_t ops = sOps[k];
-            const uint32_t a = ops & 0xffff, b = ops >> 16;
-            const Real pk = sCoef[k];
-            const Real fa = bF[a], fb = bF[b], ga = bB[a], gb = bB[b];
-            if (!Scaled) {
-              t = pk * (fa * gb + ga * fb);
-            } else {
-              const int Eo = sEi[own], ea = sEi[a], eb = sEi[b];
-              if (Eo >= kOpen / 2) {
-                const int K = Eo - kOpen;
-                Real m;
-                int ee;
-                if (fa != Real(0) && gb != Real(0)) { xprod(pk, fa, ea, gb, eb, &m, &ee); t += ldexp_r(m, ee - K); }
-                if (ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ee); t += ldexp_r(m, ee - K); }
-              } else if (Eo != kUndef && ea != kUndef && eb != kUndef) {
-                // fixed exponent: the term formed in FP64 (wide enough for any
-                // product of the stored values), rescaled to it
-                t = (Real)ldexp((double)pk * ((double)fa * gb + (double)ga * fb), ea + eb - Eo);
-              }
-            }
-          }
-          bool last;
-          t = row_seg_sum(t, own, &last);
-          if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
-        }
-        wsync<Gws>();
-      }
-      // row 0: terms from the child branches (whale!/whalewgd!, src/core.jl:95-128)
-      auto row0 = [&](const SegInfo& sg, uint32_t i) {
-        const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
-        Real v = 0;
-        if (fast0 && sg.kind != WHALE_NODE_LEAF) {
-          // speciation: the split terms are in sAcc
-          const Real sp = sAcc[i];
-          sAcc[i] = 0;
-          const Real ef = NC.etop[sg.c0], eg = NC.etop[sg.c1];
-          if (!Scaled) {
-            v = sp + (bF[i] * eg + bB[i] * ef);
-          } else {
-            const int Eg = sEi[i];
-            if (Eg >= kOpen / 2) {
-              int e2;
-              v = ldexp_r(frexp_r(sp, &e2), -kBirth);
-              sEi[i] = Eg - kOpen + e2 + kBirth;
-              sFlag[i] = kYoung;
-            } else if (Eg != kUndef) {
-              v = (bF[i] * eg + bB[i] * ef) + sp;
-            }
-          }
-        } else if (sg.kind == WHALE_NODE_LEAF) {
-          if (r1 == r0) {  // a clade of one gene (every larger clade has a split)
-            const Real lp = NC.leafP[sg.node];
-            if (Scaled) {
-              const int Eg = sEi[i];
-              if (Eg == kUndef) {
-                int ex;
-                v = frexp_r(lp, &ex);
-                sEi[i] = ex;
-                sFlag[i] = kYoung;
-              } else {
-                v = ldexp_r(lp, -Eg);
-              }
-            } else {
-              v = lp;
-            }
-          }
-        } else if (!Scaled) {
-          if (sg.kind == WHALE_NODE_WGD) {
-            const Real qq = NC.q[sg.wgd];
-            const Real ef = NC.etop[sg.c0];
-            Real r = 0;
-            for (uint32_t k = r0; k < r1; ++k) {
-              const uint32_t ops = sOps[k];
-              r += sCoef[k] * bF[ops & 0xffff] * bF[ops >> 16];
-            }
-            v = r * qq + ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]);
-          } else {
-            const Real ef = NC.etop[sg.c0], eg = NC.etop[sg.c1];
-            Real sp = 0;
-            for (uint32_t k = r0; k < r1; ++k) {
-              const uint32_t ops = sOps[k];
-              const uint32_t a = ops & 0xffff, b = ops >> 16;
-              sp += sCoef[k] * (bF[a] * bB[b] + bB[a] * bF[b]);
-            }
-            v = sp + (bF[i] * eg + bB[i] * ef);
-          }
-        } else {
-          // scaled: explicit exponents (the operands come from other segments)
-          const bool wgd = sg.kind == WHALE_NODE_WGD;
-          const Real qq = wgd ? NC.q[sg.wgd] : Real(1);
-          const Real ef = NC.etop[sg.c0];
-          const Real eg = wgd ? Real(0) : NC.etop[sg.c1];
-          const int Eg = sEi[i];
-          if (Eg != kUndef) {
-            // exponent fixed: the split terms, rescaled to it, summed in FP64
-            // (wide enough for any product of the stored values)
-            double sacc = 0.0;
-            for (uint32_t k = r0; k < r1; ++k) {
-              const uint32_t ops = sOps[k];
-              const uint32_t a = ops & 0xffff, b = ops >> 16;
-              const int ea = sEi[a], eb = sEi[b];
-              if (ea == kUndef || eb == kUndef) continue;  // parts without values yet
-              const double fa = bF[a], fb = bF[b];
-              const double t = wgd ? (double)qq * fa * fb : fa * (double)bB[b] + (double)bB[a] * fb;
-              sacc += ldexp((double)sCoef[k] * t, ea + eb - Eg);
-            }
-            const Real base = wgd ? ((Real(1) - qq) * bF[i] + (Real(2) * qq) * ef * bF[i]) : (bF[i] * eg + bB[i] * ef);
-            v = base + (Real)sacc;
-          } else {
-            // first values of this clade: explicit exponents, then its own
-            XSum<Real> xs;
-            for (uint32_t k = r0; k < r1; ++k) {
-              const uint32_t ops = sOps[k];
-              const uint32_t a = ops & 0xffff, b = ops >> 16;
-              const Real pk = sCoef[k];
-              const Real fa = bF[a], fb = bF[b];
-              const Real ga = wgd ? fa : bB[a], gb = wgd ? fb : bB[b];
-              const int ea = sEi[a], eb = sEi[b];
-              Real m;
-              int ex;
-              if (fa != Real(0) && gb != Real(0)) { xprod(pk * qq, fa, ea, gb, eb, &m, &ex); xs.add(m, ex); }
-              if (!wgd && ga != Real(0) && fb != Real(0)) { xprod(pk, ga, ea, fb, eb, &m, &ex); xs.add(m, ex); }
-            }
-            if (xs.k != kUndef) {
-              int ex;
-              v = ldexp_r(frexp_r(xs.s, &ex), -kBirth);
-              sEi[i] = xs.k + ex + kBirth;
-              sFlag[i] = kYoung;
-            }
-          }
-        }
-        bA[i] = v;
-        if (StoreAll || sg.R == 1) ell[sg.ell + (i - sg.item0)] = v;
-      };
-      // row 0 of every item; the first kCache per lane also fill the register cache
-      ItemRow cache[kCache];
-#pragma unroll
-      for (int c = 0; c < kCache; ++c) {
-        if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
-          const uint32_t i = lane + c * kWG;
-          const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
-          if (i < n) {
-            cache[c] = item_row(sg, i);
-            row0(sg, i);
-          }
-        }
-      }
-      for (uint32_t base = kCache * kWG; base < n; base += kWG) {
-        const uint32_t i = base + lane;
-        const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
-        if (i < n) row0(sg, i);
-      }
-      wsync<Gws>();
-      // scaled forms: coefficients for the exponents now fixed.  A record whose
-      // clade is still open keeps p (the open clade sums explicit exponents).
-      auto refresh = [&]() {
-        for (uint32_t k = lane; k < nrec; k += kWG) {
-          const uint32_t ops = sOps[k];
-          const int Eg = sEi[sOwn[k]];
-          const Real p = sP[k];
-          const Real c = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
-          sCoef[k] = Eg == kUndef ? p : c;
-          if (StoreAll) W.coef[rb + k] = c;
-        }
-      };
-      // items still without values (open clades with splits): they take the
-      // explicit-exponent path until their first value
-      auto count_open = [&]() -> uint32_t {
-        uint32_t cnt = 0;
-        for (uint32_t base = 0; base < n; base += kWG) {
-          const uint32_t i = base + lane;
-          const bool open = i < n && (sFlag[i] & kPending) && sRO[i + 1] > sRO[i];
-          cnt += (uint32_t)__popcll(__ballot(open));
-        }
-        return cnt;
-      };
-      uint32_t n_open = 0;
-      if (Scaled) {
-        refresh();
-        wsync<Gws>();
-        n_open = count_open();
-      }
-      // the first RC records per lane in registers: operands, owner, coefficient
-      // (row-invariant until the coefficients are refreshed)
-      uint32_t rops[RC], rown[RC];
-      Real rcoef[RC];
-      RunMask rm[RC];
-      auto load_recs = [&]() {
-#pragma unroll
-        for (int c = 0; c < RC; ++c) {
-          const uint32_t k = lane + c * kWG;
-          const bool in = k < nrec;
-          rops[c] = in ? sOps[k] : 0u;
-          rown[c] = in ? (uint32_t)sOwn[k] : kNone;
-          rcoef[c] = in ? sCoef[k] : Real(0);
-          rm[c] = run_mask(rown[c]);
-        }
-      };
-      load_recs();
-      // rows 1..R-1: within_branch! (src/core.jl:83-92)
-      Real* cur = bA;
-      Real* nxt = bB;
-      for (uint32_t r = 1; r < S.r_max; ++r) {
-        bool dirty = false;
-        // split sums of every item, one record per lane (lane-balanced however
-        // the records are spread over the items); the cached records' operand
-        // reads are all issued before any is used
-        {
-          Real va[RC], vb[RC];
-#pragma unroll
-          for (int c = 0; c < RC; ++c) {
-            if ((uint32_t)c * kWG < nrec) {
-              va[c] = cur[rops[c] & 0xffff];
-              vb[c] = cur[rops[c] >> 16];
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < RC; ++c) {
-            if ((uint32_t)c * kWG < nrec) {  // uniform: the row sums use DPP
-              const Real t = run_sum(rcoef[c] * va[c] * vb[c], rm[c]);
-              if (rm[c].ends() && t != Real(0)) lds_add(&sAcc[rown[c]], t);
-            }
-          }
-        }
-        for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: the row sums use DPP
-          const uint32_t k = base + lane;
-          uint32_t own = kNone;
-          Real t = 0;
-          if (k < nrec) {
-            const uint32_t ops = sOps[k];
-            own = sOwn[k];
-            t = sCoef[k] * cur[ops & 0xffff] * cur[ops >> 16];
-          }
-          bool last;
-          t = row_seg_sum(t, own, &last);
-          if (last && own != kNone && t != Real(0)) lds_add(&sAcc[own], t);
-        }
-        wsync<Gws>();
-        // a young clade past 2^kBump is renormalised in place (its only values are
-        // this branch's rows); its exponent is raised after the row, as other items
-        // of this row still read the clade's previous row at the old scale
-        auto bump_check = [&](const ItemRow& it, uint32_t i, Real& v, uint8_t fl) {
-          if (Scaled && sizeof(Real) == sizeof(float) && (fl & kYoung) &&
-              fabsf((float)v) > (float)(1ull << kBump) && isfinite((float)v)) {
-            int ex;
-            (void)frexp_r(v, &ex);
-            const int sh = ex + kBirth;
-            v = ldexp_r(v, -sh);
-            if (StoreAll)
-              for (uint32_t q = 0; q < r; ++q) {
-                Real* p = ell + it.cell + (size_t)q * it.stride;
-                *p = ldexp_r(*p, -sh);
-              }
-            bF[i] = (Real)sh;  // bF is free after row 0
-            sFlag[i] = fl | kShifted;
-            dirty = true;
-          }
-        };
-        auto update = [&](const ItemRow& it, uint32_t i) {
-          Real v = cur[i];
-          if (r < it.R) {
-            Real ph, ps;
-            slice<Real, TabL>(M, sTab, it.tab + r, ph, ps);
-            const uint8_t fl = Scaled ? sFlag[i] : 0;
-            const uint32_t r0 = sRO[i], r1 = sRO[i + 1];
-            if (!Scaled || !(fl & kPending) || r1 == r0) {
-              v = ph * v + ps * sAcc[i];
-              bump_check(it, i, v, fl);
-            } else {
-              // clade still open: sum with explicit exponents (coefficients hold p)
-              XSum<Real> xs;
-              for (uint32_t k = r0; k < r1; ++k) {
-                const uint32_t ops = sOps[k];
-                const uint32_t a = ops & 0xffff, b = ops >> 16;
-                const Real va = cur[a], vb = cur[b];
-                if (va == Real(0) || vb == Real(0)) continue;
-                Real m;
-                int ex;
-                xprod(sCoef[k], va, sEi[a], vb, sEi[b], &m, &ex);
-                xs.add(m, ex);
-              }
-              if (xs.k != kUndef) {
-                int ex;
-                v = ldexp_r(frexp_r(ps * xs.s, &ex), -kBirth);
-                sEi[i] = xs.k + ex + kBirth;  // readers of row r-1 see 0 here and skip it
-                sFlag[i] = kYoung;
-                dirty = true;
-              }
-            }
-            if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
-          }
-          nxt[i] = v;
-          sAcc[i] = 0;
-        };
-        if (!Scaled || n_open == 0) {
-          // no clade of the stage is still open: every cached item takes the
-          // plain update; all slots' LDS reads are issued before any is used
-          Real v0[kCache], ac[kCache], ph[kCache], ps[kCache];
-          uint8_t fl[kCache];
-#pragma unroll
-          for (int c = 0; c < kCache; ++c) {
-            if ((uint32_t)c * kWG < n) {
-              const uint32_t i = lane + c * kWG;
-              const uint32_t ii = i < n ? i : 0u;
-              const bool act = i < n && r < cache[c].R;
-              slice<Real, TabL>(M, sTab, act ? cache[c].tab + r : 0u, ph[c], ps[c]);
-              v0[c] = cur[ii];
-              ac[c] = sAcc[ii];
-              fl[c] = (Scaled && sizeof(Real) == sizeof(float)) ? sFlag[ii] : 0;
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < kCache; ++c) {
-            if ((uint32_t)c * kWG < n) {
-              const uint32_t i = lane + c * kWG;
-              if (i < n) {
-                const ItemRow& it = cache[c];
-                Real v = v0[c];
-                if (r < it.R) {
-                  v = ph[c] * v + ps[c] * ac[c];
-                  bump_check(it, i, v, fl[c]);
-                  if (StoreAll || r == it.R - 1) ell[it.cell + (size_t)r * it.stride] = v;
-                }
-                nxt[i] = v;
-                sAcc[i] = 0;
-              }
-            }
-          }
-        } else {
-#pragma unroll
-          for (int c = 0; c < kCache; ++c) {
-            const uint32_t i = lane + c * kWG;
-            if (i < n) update(cache[c], i);
-          }
-        }
-        for (uint32_t base = kCache * kWG; base < n; base += kWG) {  // uniform: lane shuffles
-          const uint32_t i = base + lane;
-          const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
-          if (i < n) update(it, i);
-        }
-        wsync<Gws>();
-        if (Scaled && __any(dirty)) {
-          for (uint32_t i = lane; i < n; i += kWG) {
-            if (sFlag[i] & kShifted) {
-              sEi[i] += (int)bF[i];
-              sFlag[i] &= (uint8_t)~kShifted;
-            }
-          }
-          wsync<Gws>();
-          refresh();
-          wsync<Gws>();
-          n_open = count_open();
-          load_recs();
-        }
-        Real* t = cur;
-        cur = nxt;
-        nxt = t;
-      }
-      // exponents of this stage's clades, for the stages above and ℓ download
-      if (Scaled)
-        for (uint32_t i = lane; i < n; i += kWG) E[B.it_gid[i0 + i]] = sEi[i];
-    }
-    // ---------------- root: all clades, by size level (whaleroot!, src/core.jl:139-158)
-    {
-      wave_sync_mem();
-      const uint32_t n = F.n_clades;
-      const uint32_t i0 = F.root_item0;
-      const SegDev sg = B.seg[F.root_seg];
-      const NodeDev& nd = M.node[M.root];
-      const double eta = M.par[3 * M.n + M.n_wgd];
-      const double er = eps_top(M, M.root);
-      const double xi = 1.0 - (1.0 - eta) * er;
-      const Real cA = (Real)((1.0 - eta) * xi / eta);
-      const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
-      const Real ef = NC.etop[nd.child0], eg = NC.etop[nd.child1];
-      const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
-      // records staged in LDS when they fit the wave's region
-      const bool inl = fwd_root_bytes(sizeof(Real), Scaled, n, nrec) <= L.wave_bytes;
-      const FwdRoot<Real> T(wb, n, inl ? nrec : 0);
-      Real* const bA = T.bA;
-      Real* const bF = T.bF;
-      Real* const bB = T.bB;
-      Real* const sCoef = T.sCoef;
-      uint32_t* const sOps = T.sOps;
-      uint16_t* const sOwn = T.sOwn;
-      uint32_t* const sRO = T.sRO;
-      int32_t* const sEi = T.sEi;
-      for (uint32_t i = lane; i < n; i += kWG) {
-        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
-        bF[i] = cf >= 0 ? ell[cf] : Real(0);
-        bB[i] = cg >= 0 ? ell[cg] : Real(0);
-        sRO[i] = B.it_rec[i0 + i] - rb;
-        if (Scaled) sEi[i] = E[i];  // root items are the clades in order
-      }
-      if (lane == 0) sRO[n] = nrec;
-      if (inl) {
-#pragma unroll 4
-        for (uint32_t k = lane; k < nrec; k += kWG) {
-          sOps[k] = B.rec_ops[rb + k];
-          sOwn[k] = B.rec_own[rb + k];
-          sCoef[k] = (Real)B.rec_p[rb + k];
-        }
-      }
-      const LevelBounds LB(B, F, lane);
-      // bA accumulates each clade's root value: it starts at the direct term
-      // cB·(F(γ)ε_g + G(γ)ε_f) (zero for a clade without a value below the
-      // root), then the level passes add the split terms of its records.
-      for (uint32_t i = lane; i < n; i += kWG) bA[i] = cB * (bF[i] * eg + bB[i] * ef);
-      wsync<Gws>();
-      // record k of a level: operands, p, owner
-      auto rrec = [&](uint32_t k, uint32_t& ops, Real& p, uint32_t& own) {
-        if (inl) { ops = sOps[k]; p = sCoef[k]; own = sOwn[k]; }
-        else { ops = B.rec_ops[rb + k]; p = (Real)B.rec_p[rb + k]; own = B.rec_own[rb + k]; }
-      };
-      // Scaled forms: a clade still without an exponent at its level ("open")
-      // takes the largest exponent of its split terms.  The first record pass
-      // raises sEi[own] to kOpen + that exponent (above every real exponent,
-      // so the open owners stay recognisable); the second adds each term
-      // rescaled to its owner's exponent; the item pass normalises the open
-      // clades' sums and fixes their exponents.
-      constexpr int kOpen = 1 << 28;
-      for (uint32_t Lv = 0; Lv < F.n_lvl; ++Lv) {
-        const uint32_t a0 = LB.at(B, F, Lv), a1 = LB.at(B, F, Lv + 1);
-        const uint32_t k0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sRO[a0]);
-        const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sRO[a1]);
-        if (k0 == k1) continue;  // clades without splits keep their direct term
-        if (Scaled) {
-          for (uint32_t k = k0 + lane; k < k1; k += kWG) {
-            uint32_t ops, own;
-            Real pk;
-            rrec(k, ops, pk, own);
-            const int Eo = sEi[own];
-            if (Eo != kUndef && Eo < kOpen / 2) continue;  // exponent fixed below the root
-            const uint32_t x = ops & 0xffff, y = ops >> 16;
-            const int ex_ = sEi[x], ey = sEi[y];
-            int emax = kUndef;
-            Real m;
-            int ee;
-            if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
-            if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
-            if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); int em; (void)frexp_r(m, &em); emax = max(emax, ee + em); }
-            if (emax != kUndef) atomicMax(&sEi[own], emax + kOpen);
-          }
-          wsync<Gws>();
-        }
-        for (uint32_t base = k0; base < k1; base += kWG) {  // uniform: the run sums use DPP
-          const uint32_t k = base + lane;
-          uint32_t own = kNone;
-          Real t = 0;
-          if (k < k1) {
-            uint32_t ops;
-            Real pk;
-            rrec(k, ops, pk, own);
-            const uint32_t x = ops & 0xffff, y = ops >> 16;
-            if (!Scaled) {
-              t = pk * (cA * bA[x] * bA[y] + cB * (bF[x] * bB[y] + bB[x] * bF[y]));
-            } else {
-              const int Eo = sEi[own], ex_ = sEi[x], ey = sEi[y];
-              if (Eo >= kOpen / 2) {
-                // open owner: each product as (mantissa, exponent) against its largest
-                const int K = Eo - kOpen;
-                Real m;
-                int ee;
-                if (bA[x] != Real(0) && bA[y] != Real(0)) { xprod(pk * cA, bA[x], ex_, bA[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
-                if (bF[x] != Real(0) && bB[y] != Real(0)) { xprod(pk * cB, bF[x], ex_, bB[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
-                if (bB[x] != Real(0) && bF[y] != Real(0)) { xprod(pk * cB, bB[x], ex_, bF[y], ey, &m, &ee); t += ldexp_r(m, ee - K); }
-              } else if (Eo != kUndef && ex_ != kUndef && ey != kUndef) {
-                // fixed exponent: the term rescaled to it, formed in FP64 (wide
-                // enough for any product of the stored values)
-                const double tt = (double)cA * bA[x] * bA[y] + (double)cB * ((double)bF[x] * bB[y] + (double)bB[x] * bF[y]);
-                t = (Real)ldexp((double)pk * tt, ex_ + ey - Eo);
-              }
-            }
-          }
-          bool last;
-          t = row_seg_sum(t, own, &last);
-          if (last && own != kNone && t != Real(0)) lds_add(&bA[own], t);
-        }
-        wsync<Gws>();
-        if (Scaled) {
-          for (uint32_t i = a0 + lane; i < a1; i += kWG) {
-            const int Eo = sEi[i];
-            if (Eo >= kOpen / 2) {
-              int e2;
-              const Real mm = frexp_r(bA[i], &e2);
-              bA[i] = ldexp_r(mm, -kBirth);
-              sEi[i] = Eo - kOpen + e2 + kBirth;
-            }
-          }
-          wsync<Gws>();
-        }
-      }
-      for (uint32_t i = lane; i < n; i += kWG) {
-        ell[sg.ell + i] = bA[i];
-        if (Scaled) E[i] = sEi[i];
-      }
-      if (lane == 0) {
-        const Real Lr = bA[n - 1];
-        double ll;
-        if (Lr > Real(0) && isfinite((double)Lr)) {
-          ll = log((double)Lr);
-          if (Scaled) ll += (double)sEi[n - 1] * 0.69314718055994530942;
-        } else {
-          ll = -INFINITY;
-          if (Scaled && sizeof(Real) == sizeof(float)) {
-            // overflow, or everything underflowed: redo this family in FP64 (a true zero stays -Inf there)
-            W.flag[f] = 1u;
-            W.fb[1 + atomicAdd(W.fb, 1u)] = f;
-          }
-        }
-        W.fam_ll[f] = ll;
-      }
-      if (Scaled && StoreAll) {
-        // final coefficients of the root records, for the reverse sweep
-        for (uint32_t k = lane; k < nrec; k += kWG) {
-          const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
-          const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
-          const int Eg = sEi[inl ? sOwn[k] : B.rec_own[rb + k]];
-          W.coef[rb + k] = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
-        }
-      }
-    }
+    fwd_family<Real, Scaled, StoreAll, TabL, Gws>(B, M, W, L, f, wb, sTab, NC, lane, W.ell + B.fam[f].ell0,
+                                                  W.coef);
   }
 }
 
@@ -1268,6 +1280,484 @@ __device__ __forceinline__ ItemTop item_top(const SegInfo& sg, uint32_t i) {
   return t;
 }
 
+// The reverse sweep of family f by one wave, from its stored ℓ (`ell`,
+// relative cells) and record coefficients (`coefs`, absolute records; the
+// plain form takes p), writing ī through `ibar` (relative cells) and adding
+// into the accumulators accPS (LDS, with TabL) and accX.
+template <typename Real, bool Scaled, bool TabL, bool Gws>
+__device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M, const WorkDev<Real>& W,
+                                           const LdsCfg& L, uint32_t f, unsigned char* const wb, Real* sTab,
+                                           Real* const accPS, double* const accX, const NodeC<Real>& NC,
+                                           uint32_t lane, const Real* ell, Real* ibar, const Real* coefs) {
+  constexpr int RC = 2;  // records per lane held in registers (128 per stage)
+  const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
+  const FamDev F = B.fam[f];
+  auto coef_at = [&](uint32_t k) -> Real { return Scaled ? coefs[k] : (Real)B.rec_p[k]; };
+  // ---------------- root
+  {
+    const uint32_t n = F.n_clades;
+    const uint32_t i0 = F.root_item0;
+    const SegDev sg = B.seg[F.root_seg];
+    const NodeDev& nd = M.node[M.root];
+    const double eta = M.par[3 * M.n + M.n_wgd];
+    const double er = eps_top(M, M.root);
+    const double xi = 1.0 - (1.0 - eta) * er;
+    const Real cA = (Real)((1.0 - eta) * xi / eta);
+    const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
+    const Real ef = NC.etop[nd.child0], eg = NC.etop[nd.child1];
+    const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
+    const bool inl = bwd_root_bytes(sizeof(Real), n, nrec) <= L.wave_bytes;
+    const BwdRoot<Real> T(wb, n, inl ? nrec : 0);
+    Real* const bA = T.bA;
+    Real* const bB = T.bB;
+    Real* const bV = T.bV;
+    Real* const bG = T.bG;
+    Real* const sCoef = T.sCoef;
+    uint32_t* const sOps = T.sOps;
+    uint16_t* const sOwn = T.sOwn;
+    uint32_t* const sRO = T.sRO;
+    const LevelBounds LB(B, F, lane);
+    for (uint32_t i = lane; i < n; i += kWG) {
+      const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
+      bV[i] = ell[sg.ell + i];
+      bB[i] = cf >= 0 ? ell[cf] : Real(0);  // child0 tops
+      bG[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops
+      bA[i] = 0;
+      sRO[i] = B.it_rec[i0 + i] - rb;
+    }
+    if (lane == 0) sRO[n] = nrec;
+    if (inl)
+      for (uint32_t k = lane; k < nrec; k += kWG) {
+        sOps[k] = B.rec_ops[rb + k];
+        sOwn[k] = B.rec_own[rb + k];
+        sCoef[k] = coef_at(rb + k);
+      }
+    wsync<Gws>();
+    // record k: operands, coefficient, owning clade
+    auto rec = [&](uint32_t k, uint32_t& ops, Real& c, uint32_t& own) {
+      if (inl) { ops = sOps[k]; c = sCoef[k]; own = sOwn[k]; }
+      else { ops = B.rec_ops[rb + k]; c = coef_at(rb + k); own = B.rec_own[rb + k]; }
+    };
+    const Real Lr = bV[n - 1];
+    const bool ok = Lr > Real(0) && isfinite((double)Lr);
+    if (ok) {
+      if (lane == 0) bA[n - 1] = Real(1) / Lr;
+      wsync<Gws>();
+      // d logL / d ℓ of each root clade, larger clades first, scattered into
+      // the parts; a level's records (those of its clades) one per lane
+      for (int Lv = (int)F.n_lvl - 1; Lv >= 0; --Lv) {
+        const uint32_t a0 = LB.at(B, F, (uint32_t)Lv), a1 = LB.at(B, F, (uint32_t)Lv + 1);
+        const uint32_t k1 = sRO[a1];
+        for (uint32_t k = sRO[a0] + lane; k < k1; k += kWG) {
+          uint32_t ops, own;
+          Real c;
+          rec(k, ops, c, own);
+          const Real t = cA * bA[own] * c;
+          if (t == Real(0)) continue;
+          const uint32_t x = ops & 0xffff, y = ops >> 16;
+          lds_add(&bA[x], t * bV[y]);
+          lds_add(&bA[y], t * bV[x]);
+        }
+        wsync<Gws>();
+      }
+      Real sA = 0, sBc = 0, sF = 0, sG = 0;
+      for (uint32_t k = lane; k < nrec; k += kWG) {
+        uint32_t ops, own;
+        Real c;
+        rec(k, ops, c, own);
+        const Real g = bA[own];
+        const uint32_t x = ops & 0xffff, y = ops >> 16;
+        sA += g * (c * bV[x] * bV[y]);
+        sBc += g * (c * (bB[x] * bG[y] + bG[x] * bB[y]));
+      }
+      for (uint32_t i = lane; i < n; i += kWG) {
+        const Real g = bA[i];
+        sBc += g * (bB[i] * eg + bG[i] * ef);
+        sF += g * bB[i];
+        sG += g * bG[i];
+      }
+      sA = wave_sum(sA);
+      sBc = wave_sum(sBc);
+      sF = wave_sum(sF);
+      sG = wave_sum(sG);
+      if (lane == 0) {
+        acc_add<TabL>(accX + AL.rootA, (double)sA);
+        acc_add<TabL>(accX + AL.rootB, (double)sBc);
+        acc_add<TabL>(accX + AL.eps + nd.child1, (double)cB * (double)sF);
+        acc_add<TabL>(accX + AL.eps + nd.child0, (double)cB * (double)sG);
+      }
+      // ī of the child tops: child0 slot takes ℓ of child1 tops and vice versa
+      wsync<Gws>();
+      for (int slot = 0; slot < 2; ++slot) {
+        const Real* other = slot == 0 ? bG : bB;
+        const Real eo = slot == 0 ? eg : ef;
+        for (uint32_t i = lane; i < n; i += kWG) bV[i] = bA[i] * eo;
+        wsync<Gws>();
+        for (uint32_t k = lane; k < nrec; k += kWG) {
+          uint32_t ops, own;
+          Real c;
+          rec(k, ops, c, own);
+          const Real g = bA[own] * c;
+          if (g == Real(0)) continue;
+          const uint32_t x = ops & 0xffff, y = ops >> 16;
+          lds_add(&bV[x], g * other[y]);
+          lds_add(&bV[y], g * other[x]);
+        }
+        wsync<Gws>();
+        for (uint32_t i = lane; i < n; i += kWG) ibar[sg.ibar + slot * n + i] = cB * bV[i];
+        wsync<Gws>();
+      }
+    } else {
+      for (uint32_t i = lane; i < 2 * n; i += kWG) ibar[sg.ibar + i] = 0;
+    }
+  }
+  // ---------------- non-root stages, top down
+  for (int s = (int)F.n_stages - 1; s >= 0; --s) {
+    const StageDev S = B.stage[F.stage0 + s];
+    const uint32_t n = S.n_items, i0 = S.item0;
+    const uint32_t rb = S.rec0, nrec = S.n_recs;
+    const BwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
+    Real* bA = T.bA;  // d logL / d ℓ, current row
+    Real* bB = T.bB;  // next row (and ī sums at row 0)
+    Real* const bV = T.bV;  // ℓ of the row below / child0 tops
+    Real* const bG = T.bG;  // child1 tops; ψ·ā per item in the row loop
+    Real* const sFo = T.sFo;  // per-item sums and factors of the row-0 step
+    Real* const sCoef = T.sCoef;
+    uint32_t* const sOps = T.sOps;
+    uint16_t* const sOwn = T.sOwn;
+    uint32_t* const sRO = T.sRO;
+    uint16_t* const sSegL = T.sSegL;
+    wave_sync_mem();  // ī written by the stage above
+    SegTab ST;
+    ST.load(B, M, S.seg0, S.n_segs, lane);
+    for (uint32_t i = lane; i < n; i += kWG) {
+      bA[i] = ibar[B.it_bar[i0 + i]];
+      bV[i] = 0;
+      sRO[i] = B.it_rec[i0 + i] - rb;
+      sSegL[i] = (uint16_t)(B.it_seg[i0 + i] - S.seg0);
+    }
+    if (lane == 0) sRO[n] = nrec;
+#pragma unroll 4
+    for (uint32_t k = lane; k < nrec; k += kWG) {
+      sOps[k] = B.rec_ops[rb + k];
+      sOwn[k] = B.rec_own[rb + k];
+      sCoef[k] = coef_at(rb + k);
+    }
+    wsync<Gws>();
+    ItemRow cache[kCacheB];
+    ItemTop tops[kCacheB];
+    // ℓ of the rows below, loaded two rows ahead into two register sets
+    // that take turns (rows r and r-1 use preA and preB)
+    Real preA[kCacheB], preB[kCacheB];
+    const int rtop = (int)S.r_max - 1;
+#pragma unroll
+    for (int c = 0; c < kCacheB; ++c) {
+      preA[c] = 0;
+      preB[c] = 0;
+      if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
+        const uint32_t i = lane + c * kWG;
+        const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+        if (i < n) {
+          cache[c] = item_row(sg, i);
+          tops[c] = item_top(sg, i);
+          const ItemRow& it = cache[c];
+          if (rtop >= 1 && rtop < (int)it.R) preA[c] = ell[it.cell + (size_t)(rtop - 1) * it.stride];
+          if (rtop >= 2 && rtop - 1 < (int)it.R) preB[c] = ell[it.cell + (size_t)(rtop - 2) * it.stride];
+        }
+      }
+    }
+    // runs of items sharing a segment (table row), for the d/dφ, d/dψ sums
+    RunMask im[kCacheB];
+#pragma unroll
+    for (int c = 0; c < kCacheB; ++c) {
+      if ((uint32_t)c * kWG < n) {
+        const uint32_t i = lane + c * kWG;
+        im[c] = run_mask(i < n ? cache[c].tab : kNone);
+      }
+    }
+    Real* const sT = bG;  // ψ_r·ā(r) per item, for the record pass
+    // the first RC records per lane in registers: operands, owner,
+    // coefficient, and the table row of the owner's segment with the runs
+    // of records sharing it (the records of a segment are consecutive)
+    uint32_t rops[RC], rown[RC], rtab[RC];
+    Real rcoef[RC];
+    RunMask rs[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      const uint32_t k = lane + c * kWG;
+      const bool in = k < nrec;
+      rops[c] = in ? sOps[k] : 0u;
+      rown[c] = in ? (uint32_t)sOwn[k] : kNone;
+      rcoef[c] = in ? sCoef[k] : Real(0);
+      if ((uint32_t)c * kWG < nrec) {  // uniform: lane shuffles
+        const uint32_t t = ST.tab(B, M, in ? (uint32_t)sSegL[rown[c]] : 0u);
+        rtab[c] = in ? t : kNone;
+        rs[c] = run_mask(rtab[c]);
+      }
+    }
+    // d/dψ_r: Σ over a segment's records of ā(r)[owner]·c·ℓ(γ1)ℓ(γ2), one
+    // atomic per run of records (records past their segment's rows have
+    // ψ_r·ā = 0 and add nothing)
+    auto add_psi = [&](uint32_t key, int r, Real x) {
+      if (TabL) lds_add(&accPS[AL.psi + key + r], x);
+      else acc_add<false>(W.acc + AL.psi + key + r, (double)x);
+    };
+    // Per row r (top down) two item passes and one record pass:
+    //   A(r)    items: ℓ of row r-1 (prefetched), φ_r·ā(r) into the other
+    //           buffer (the start of ā(r-1)) and ψ_r·ā(r) for the records
+    //   REC(r)  records: the ψ part scattered into the two parts' ā(r-1),
+    //           and the owner's split sum (forcing of d/dψ_r)
+    //   C(r)    items: d/dφ_r, d/dψ_r per segment
+    // C(r) and A(r-1) share one pass over the items.
+    // A(x) for item i from ā(x) = g into dst; pv = ℓ(x-1)
+    // (branch-free: past its segment's rows an item passes ā through and
+    // its records' forcing is zero; pv is then a finite stale value)
+    auto item_a = [&](const int x, const ItemRow& it, uint32_t i, Real g, Real pv, Real* dst) {
+      const bool in = x < (int)it.R;
+      Real ph, ps;
+      slice<Real, TabL>(M, sTab, in ? it.tab + x : 0u, ph, ps);
+      bV[i] = pv;
+      dst[i] = in ? ph * g : g;
+      sT[i] = in ? ps * g : Real(0);
+    };
+    auto refill = [&](const int x, Real (&pre)[kCacheB]) {  // ℓ(x-3) for A(x-2)
+#pragma unroll
+      for (int c = 0; c < kCacheB; ++c) {
+        const uint32_t i = lane + c * kWG;
+        if (i < n && x >= 3 && x - 2 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(x - 3) * cache[c].stride];
+      }
+    };
+    // C(r) for item i: d/dφ_r from ā(r) = g and ℓ(r-1) = v (all lanes take part)
+    auto item_c = [&](const int r, const ItemRow& it, bool on, Real g, Real v, const RunMask& rmk) {
+      Real cphi = on ? g * v : Real(0);
+      cphi = run_sum(cphi, rmk);
+      if (rmk.ends() && on && cphi != Real(0)) {
+        const uint32_t key = it.tab;
+        if (TabL) lds_add(&accPS[AL.phi + key + r], cphi);
+        else acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
+      }
+    };
+    // A(rtop) on its own
+    {
+#pragma unroll
+      for (int c = 0; c < kCacheB; ++c) {
+        const uint32_t i = lane + c * kWG;
+        if (i < n) item_a(rtop, cache[c], i, bA[i], preA[c], bB);
+      }
+      for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {  // uniform: lane shuffles
+        const uint32_t i = base + lane;
+        const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
+        if (i < n)
+          item_a(rtop, it, i, bA[i], rtop < (int)it.R ? ell[it.cell + (size_t)(rtop - 1) * it.stride] : Real(0), bB);
+      }
+      refill(rtop, preA);
+    }
+    auto rec_pass = [&](const int r) {
+      // records, one per lane; the cached records' reads are all issued
+      // before any is used
+      {
+        Real va[RC], vb[RC], g[RC], ga[RC];
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+          if ((uint32_t)c * kWG < nrec) {
+            const uint32_t o = rown[c] != kNone ? rown[c] : 0u;
+            va[c] = bV[rops[c] & 0xffff];
+            vb[c] = bV[rops[c] >> 16];
+            g[c] = rown[c] != kNone ? sT[o] : Real(0);
+            ga[c] = bA[o];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+          if ((uint32_t)c * kWG < nrec) {  // uniform: the run sums use DPP
+            const Real t = g[c] * rcoef[c];
+            if (t != Real(0)) {
+              lds_add(&bB[rops[c] & 0xffff], t * vb[c]);
+              lds_add(&bB[rops[c] >> 16], t * va[c]);
+            }
+            const Real x = run_sum(t != Real(0) ? ga[c] * rcoef[c] * va[c] * vb[c] : Real(0), rs[c]);
+            if (rs[c].ends() && x != Real(0)) add_psi(rtab[c], r, x);
+          }
+        }
+      }
+      for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: shuffles, row sums
+        const uint32_t k = base + lane;
+        const bool in = k < nrec;
+        const uint32_t own = in ? (uint32_t)sOwn[k] : 0u;
+        const uint32_t tb = ST.tab(B, M, sSegL[own]);
+        const uint32_t key = in ? tb : kNone;
+        Real x = 0;
+        if (in) {
+          const uint32_t ops = sOps[k];
+          const uint32_t a = ops & 0xffff, b = ops >> 16;
+          const Real c = sCoef[k];
+          const Real va = bV[a], vb = bV[b];
+          const Real t = sT[own] * c;
+          if (t != Real(0)) {
+            lds_add(&bB[a], t * vb);
+            lds_add(&bB[b], t * va);
+            x = bA[own] * c * va * vb;
+          }
+        }
+        bool last;
+        x = row_seg_sum(x, key, &last);
+        if (last && key != kNone && x != Real(0)) add_psi(key, r, x);
+      }
+    };
+    // REC(r), then C(r) with A(r-1); A(r-1) reads ℓ(r-2) from `pre`
+    auto row_step = [&](const int r, Real (&pre)[kCacheB]) {
+      wsync<Gws>();
+      rec_pass(r);
+      wsync<Gws>();
+      const bool more = r >= 2;  // uniform
+#pragma unroll
+      for (int c = 0; c < kCacheB; ++c) {
+        if ((uint32_t)c * kWG < n) {  // uniform: the run sums use DPP
+          const uint32_t i = lane + c * kWG;
+          const uint32_t ii = i < n ? i : 0u;
+          const Real g = bA[ii], v = bV[ii], g1 = bB[ii];
+          item_c(r, cache[c], i < n && r < (int)cache[c].R, g, v, im[c]);
+          if (more && i < n) item_a(r - 1, cache[c], i, g1, pre[c], bA);
+        }
+      }
+      for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {  // uniform: lane shuffles, DPP
+        const uint32_t i = base + lane;
+        const uint32_t ii = i < n ? i : 0u;
+        const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
+        const Real g = bA[ii], v = bV[ii], g1 = bB[ii];
+        item_c(r, it, i < n && r < (int)it.R, g, v, run_mask(i < n ? it.tab : kNone));
+        if (more && i < n)
+          item_a(r - 1, it, i, g1, r - 1 < (int)it.R ? ell[it.cell + (size_t)(r - 2) * it.stride] : Real(0), bA);
+      }
+      if (more) refill(r - 1, pre);
+      Real* t = bA;
+      bA = bB;
+      bB = t;
+    };
+    for (int r = rtop; r >= 1; r -= 2) {
+      row_step(r, preB);
+      if (r >= 2) row_step(r - 1, preA);
+    }
+    wsync<Gws>();
+    if (TabL) {
+      // move this stage's d/dφ_r, d/dψ_r into the global FP64 sums (each LDS
+      // partial then holds at most one stage's contributions per family)
+      for (uint32_t sl = 0; sl < S.n_segs; ++sl) {
+        uint32_t tab, R;
+        if (ST.inreg) {
+          tab = (uint32_t)__builtin_amdgcn_readlane((int)ST.mine.tab, (int)sl);
+          R = (uint32_t)__builtin_amdgcn_readlane((int)ST.mine.R, (int)sl);
+        } else {
+          const SegInfo g = seg_info(B, M, S.seg0 + sl);
+          tab = g.tab;
+          R = g.R;
+        }
+        for (uint32_t r = 1 + lane; r < R; r += kWG) {
+          const Real a = __hip_atomic_exchange(&accPS[AL.phi + tab + r], Real(0), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+          const Real b = __hip_atomic_exchange(&accPS[AL.psi + tab + r], Real(0), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (a != Real(0)) unsafeAtomicAdd(W.acc + AL.phi + tab + r, (double)a);
+          if (b != Real(0)) unsafeAtomicAdd(W.acc + AL.psi + tab + r, (double)b);
+        }
+      }
+    }
+    // row 0: adjoints of the child tops, ε and q
+    // pass over all items: cached tops for the first kCacheB per lane
+    auto each = [&](auto&& fn) {
+#pragma unroll
+      for (int c = 0; c < kCacheB; ++c) {
+        const uint32_t i = lane + c * kWG;
+        if (i < n) fn(tops[c], i);
+      }
+      for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {
+        const uint32_t i = base + lane;
+        const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
+        if (i < n) fn(item_top(sg, i), i);
+      }
+    };
+    // pass over all records: the first RC per lane from registers
+    auto each_rec = [&](auto&& fn) {
+#pragma unroll
+      for (int c = 0; c < RC; ++c)
+        if ((uint32_t)c * kWG < nrec && rown[c] != kNone) fn(rops[c], rown[c], rcoef[c]);
+      for (uint32_t k = RC * kWG + lane; k < nrec; k += kWG) fn(sOps[k], (uint32_t)sOwn[k], sCoef[k]);
+    };
+    // child tops; a WGD item's one child also stands in the child1 slot, so
+    // the split terms below read the "other" top from bG for both kinds
+    bool any_wgd = false;
+    each([&](const ItemTop& tp, uint32_t i) {
+      const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
+      const Real v0 = cf >= 0 ? ell[cf] : Real(0);
+      bV[i] = v0;
+      bG[i] = tp.kind == WHALE_NODE_WGD ? v0 : (cg >= 0 ? ell[cg] : Real(0));
+      sFo[i] = 0;
+      any_wgd |= tp.kind == WHALE_NODE_WGD;
+    });
+    wsync<Gws>();
+    if (__any(any_wgd)) {
+      // WGD retention sums Σ c·ℓ(γ1)ℓ(γ2) over the child's tops, per owner
+      each_rec([&](uint32_t ops, uint32_t own, Real c) {
+        const Real t = c * bV[ops & 0xffff] * bV[ops >> 16];
+        if (t != Real(0)) lds_add(&sFo[own], t);
+      });
+      wsync<Gws>();
+    }
+    // ε̄, q̄ and the direct terms of the child0 slot; then each owner's
+    // factor of its split terms in that slot
+    each([&](const ItemTop& tp, uint32_t i) {
+      const Real g = bA[i];
+      Real fac = 0;
+      if (tp.kind == WHALE_NODE_SPECIATION) {
+        const Real eg = NC.etop[tp.c1];
+        acc_add<TabL>(accX + AL.eps + tp.c1, (double)(g * bV[i]));
+        acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * bG[i]));
+        bB[i] = g * eg;
+        fac = g;
+      } else if (tp.kind == WHALE_NODE_WGD) {
+        const Real qq = NC.q[tp.wgd];
+        const Real ef = NC.etop[tp.c0];
+        const Real ret = sFo[i];
+        acc_add<TabL>(accX + AL.q + tp.wgd, (double)(g * (ret - bV[i] + Real(2) * ef * bV[i])));
+        acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * Real(2) * qq * bV[i]));
+        bB[i] = g * ((Real(1) - qq) + Real(2) * qq * ef);
+        fac = qq * g;
+      }
+      sFo[i] = fac;
+    });
+    wsync<Gws>();
+    // split terms of the child0 slot: speciation pairs child0 parts with child1
+    // tops of the other part; a WGD pairs both parts on its one child
+    each_rec([&](uint32_t ops, uint32_t own, Real c) {
+      const Real g = sFo[own] * c;
+      if (g == Real(0)) return;
+      const uint32_t a = ops & 0xffff, b = ops >> 16;
+      lds_add(&bB[a], g * bG[b]);
+      lds_add(&bB[b], g * bG[a]);
+    });
+    wsync<Gws>();
+    each([&](const ItemTop& tp, uint32_t i) {
+      if (tp.kind == WHALE_NODE_SPECIATION || tp.kind == WHALE_NODE_WGD) ibar[tp.ib] = bB[i];
+      const bool sp = tp.kind == WHALE_NODE_SPECIATION;
+      if (sp) bB[i] = bA[i] * NC.etop[tp.c0];
+      sFo[i] = sp ? bA[i] : Real(0);
+    });
+    wsync<Gws>();
+    // child1 slot of speciation nodes
+    each_rec([&](uint32_t ops, uint32_t own, Real c) {
+      const Real g = sFo[own] * c;
+      if (g == Real(0)) return;
+      const uint32_t a = ops & 0xffff, b = ops >> 16;
+      lds_add(&bB[a], g * bV[b]);
+      lds_add(&bB[b], g * bV[a]);
+    });
+    wsync<Gws>();
+    each([&](const ItemTop& tp, uint32_t i) {
+      if (tp.kind == WHALE_NODE_SPECIATION) ibar[tp.ib + tp.nc] = bB[i];
+    });
+  }
+}
+
 template <typename Real, bool Scaled, bool TabL, bool Gws>
 __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                         int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
@@ -1317,473 +1807,8 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
     }
     first = false;
     if (Scaled && sizeof(Real) == sizeof(float) && W.flag[f]) continue;  // done by the FP64 pass
-    const FamDev F = B.fam[f];
-    const Real* ell = W.ell + F.ell0;
-    Real* ibar = W.ibar + F.ibar0;
-    auto coef_at = [&](uint32_t k) -> Real { return Scaled ? coefs[k] : (Real)B.rec_p[k]; };
-    // ---------------- root
-    {
-      const uint32_t n = F.n_clades;
-      const uint32_t i0 = F.root_item0;
-      const SegDev sg = B.seg[F.root_seg];
-      const NodeDev& nd = M.node[M.root];
-      const double eta = M.par[3 * M.n + M.n_wgd];
-      const double er = eps_top(M, M.root);
-      const double xi = 1.0 - (1.0 - eta) * er;
-      const Real cA = (Real)((1.0 - eta) * xi / eta);
-      const Real cB = (Real)(eta * (1.0 - er) / (xi * xi));
-      const Real ef = NC.etop[nd.child0], eg = NC.etop[nd.child1];
-      const uint32_t rb = B.it_rec[i0], nrec = B.it_rec[i0 + n] - rb;
-      const bool inl = bwd_root_bytes(sizeof(Real), n, nrec) <= L.wave_bytes;
-      const BwdRoot<Real> T(wb, n, inl ? nrec : 0);
-      Real* const bA = T.bA;
-      Real* const bB = T.bB;
-      Real* const bV = T.bV;
-      Real* const bG = T.bG;
-      Real* const sCoef = T.sCoef;
-      uint32_t* const sOps = T.sOps;
-      uint16_t* const sOwn = T.sOwn;
-      uint32_t* const sRO = T.sRO;
-      const LevelBounds LB(B, F, lane);
-      for (uint32_t i = lane; i < n; i += kWG) {
-        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
-        bV[i] = ell[sg.ell + i];
-        bB[i] = cf >= 0 ? ell[cf] : Real(0);  // child0 tops
-        bG[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops
-        bA[i] = 0;
-        sRO[i] = B.it_rec[i0 + i] - rb;
-      }
-      if (lane == 0) sRO[n] = nrec;
-      if (inl)
-        for (uint32_t k = lane; k < nrec; k += kWG) {
-          sOps[k] = B.rec_ops[rb + k];
-          sOwn[k] = B.rec_own[rb + k];
-          sCoef[k] = coef_at(rb + k);
-        }
-      wsync<Gws>();
-      // record k: operands, coefficient, owning clade
-      auto rec = [&](uint32_t k, uint32_t& ops, Real& c, uint32_t& own) {
-        if (inl) { ops = sOps[k]; c = sCoef[k]; own = sOwn[k]; }
-        else { ops = B.rec_ops[rb + k]; c = coef_at(rb + k); own = B.rec_own[rb + k]; }
-      };
-      const Real Lr = bV[n - 1];
-      const bool ok = Lr > Real(0) && isfinite((double)Lr);
-      if (ok) {
-        if (lane == 0) bA[n - 1] = Real(1) / Lr;
-        wsync<Gws>();
-        // d logL / d ℓ of each root clade, larger clades first, scattered into
-        // the parts; a level's records (those of its clades) one per lane
-        for (int Lv = (int)F.n_lvl - 1; Lv >= 0; --Lv) {
-          const uint32_t a0 = LB.at(B, F, (uint32_t)Lv), a1 = LB.at(B, F, (uint32_t)Lv + 1);
-          const uint32_t k1 = sRO[a1];
-          for (uint32_t k = sRO[a0] + lane; k < k1; k += kWG) {
-            uint32_t ops, own;
-            Real c;
-            rec(k, ops, c, own);
-            const Real t = cA * bA[own] * c;
-            if (t == Real(0)) continue;
-            const uint32_t x = ops & 0xffff, y = ops >> 16;
-            lds_add(&bA[x], t * bV[y]);
-            lds_add(&bA[y], t * bV[x]);
-          }
-          wsync<Gws>();
-        }
-        Real sA = 0, sBc = 0, sF = 0, sG = 0;
-        for (uint32_t k = lane; k < nrec; k += kWG) {
-          uint32_t ops, own;
-          Real c;
-          rec(k, ops, c, own);
-          const Real g = bA[own];
-          const uint32_t x = ops & 0xffff, y = ops >> 16;
-          sA += g * (c * bV[x] * bV[y]);
-          sBc += g * (c * (bB[x] * bG[y] + bG[x] * bB[y]));
-        }
-        for (uint32_t i = lane; i < n; i += kWG) {
-          const Real g = bA[i];
-          sBc += g * (bB[i] * eg + bG[i] * ef);
-          sF += g * bB[i];
-          sG += g * bG[i];
-        }
-        sA = wave_sum(sA);
-        sBc = wave_sum(sBc);
-        sF = wave_sum(sF);
-        sG = wave_sum(sG);
-        if (lane == 0) {
-          acc_add<TabL>(accX + AL.rootA, (double)sA);
-          acc_add<TabL>(accX + AL.rootB, (double)sBc);
-          acc_add<TabL>(accX + AL.eps + nd.child1, (double)cB * (double)sF);
-          acc_add<TabL>(accX + AL.eps + nd.child0, (double)cB * (double)sG);
-        }
-        // ī of the child tops: child0 slot takes ℓ of child1 tops and vice versa
-        wsync<Gws>();
-        for (int slot = 0; slot < 2; ++slot) {
-          const Real* other = slot == 0 ? bG : bB;
-          const Real eo = slot == 0 ? eg : ef;
-          for (uint32_t i = lane; i < n; i += kWG) bV[i] = bA[i] * eo;
-          wsync<Gws>();
-          for (uint32_t k = lane; k < nrec; k += kWG) {
-            uint32_t ops, own;
-            Real c;
-            rec(k, ops, c, own);
-            const Real g = bA[own] * c;
-            if (g == Real(0)) continue;
-            const uint32_t x = ops & 0xffff, y = ops >> 16;
-            lds_add(&bV[x], g * other[y]);
-            lds_add(&bV[y], g * other[x]);
-          }
-          wsync<Gws>();
-          for (uint32_t i = lane; i < n; i += kWG) ibar[sg.ibar + slot * n + i] = cB * bV[i];
-          wsync<Gws>();
-        }
-      } else {
-        for (uint32_t i = lane; i < 2 * n; i += kWG) ibar[sg.ibar + i] = 0;
-      }
-    }
-    // ---------------- non-root stages, top down
-    for (int s = (int)F.n_stages - 1; s >= 0; --s) {
-      const StageDev S = B.stage[F.stage0 + s];
-      const uint32_t n = S.n_items, i0 = S.item0;
-      const uint32_t rb = S.rec0, nrec = S.n_recs;
-      const BwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
-      Real* bA = T.bA;  // d logL / d ℓ, current row
-      Real* bB = T.bB;  // next row (and ī sums at row 0)
-      Real* const bV = T.bV;  // ℓ of the row below / child0 tops
-      Real* const bG = T.bG;  // child1 tops; ψ·ā per item in the row loop
-      Real* const sFo = T.sFo;  // per-item sums and factors of the row-0 step
-      Real* const sCoef = T.sCoef;
-      uint32_t* const sOps = T.sOps;
-      uint16_t* const sOwn = T.sOwn;
-      uint32_t* const sRO = T.sRO;
-      uint16_t* const sSegL = T.sSegL;
-      wave_sync_mem();  // ī written by the stage above
-      SegTab ST;
-      ST.load(B, M, S.seg0, S.n_segs, lane);
-      for (uint32_t i = lane; i < n; i += kWG) {
-        bA[i] = ibar[B.it_bar[i0 + i]];
-        bV[i] = 0;
-        sRO[i] = B.it_rec[i0 + i] - rb;
-        sSegL[i] = (uint16_t)(B.it_seg[i0 + i] - S.seg0);
-      }
-      if (lane == 0) sRO[n] = nrec;
-#pragma unroll 4
-      for (uint32_t k = lane; k < nrec; k += kWG) {
-        sOps[k] = B.rec_ops[rb + k];
-        sOwn[k] = B.rec_own[rb + k];
-        sCoef[k] = coef_at(rb + k);
-      }
-      wsync<Gws>();
-      ItemRow cache[kCacheB];
-      ItemTop tops[kCacheB];
-      // ℓ of the rows below, loaded two rows ahead into two register sets
-      // that take turns (rows r and r-1 use preA and preB)
-      Real preA[kCacheB], preB[kCacheB];
-      const int rtop = (int)S.r_max - 1;
-#pragma unroll
-      for (int c = 0; c < kCacheB; ++c) {
-        preA[c] = 0;
-        preB[c] = 0;
-        if ((uint32_t)c * kWG < n) {  // uniform: segment lookups are lane shuffles
-          const uint32_t i = lane + c * kWG;
-          const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
-          if (i < n) {
-            cache[c] = item_row(sg, i);
-            tops[c] = item_top(sg, i);
-            const ItemRow& it = cache[c];
-            if (rtop >= 1 && rtop < (int)it.R) preA[c] = ell[it.cell + (size_t)(rtop - 1) * it.stride];
-            if (rtop >= 2 && rtop - 1 < (int)it.R) preB[c] = ell[it.cell + (size_t)(rtop - 2) * it.stride];
-          }
-        }
-      }
-      // runs of items sharing a segment (table row), for the d/dφ, d/dψ sums
-      RunMask im[kCacheB];
-#pragma unroll
-      for (int c = 0; c < kCacheB; ++c) {
-        if ((uint32_t)c * kWG < n) {
-          const uint32_t i = lane + c * kWG;
-          im[c] = run_mask(i < n ? cache[c].tab : kNone);
-        }
-      }
-      Real* const sT = bG;  // ψ_r·ā(r) per item, for the record pass
-      // the first RC records per lane in registers: operands, owner,
-      // coefficient, and the table row of the owner's segment with the runs
-      // of records sharing it (the records of a segment are consecutive)
-      uint32_t rops[RC], rown[RC], rtab[RC];
-      Real rcoef[RC];
-      RunMask rs[RC];
-#pragma unroll
-      for (int c = 0; c < RC; ++c) {
-        const uint32_t k = lane + c * kWG;
-        const bool in = k < nrec;
-        rops[c] = in ? sOps[k] : 0u;
-        rown[c] = in ? (uint32_t)sOwn[k] : kNone;
-        rcoef[c] = in ? sCoef[k] : Real(0);
-        if ((uint32_t)c * kWG < nrec) {  // uniform: lane shuffles
-          const uint32_t t = ST.tab(B, M, in ? (uint32_t)sSegL[rown[c]] : 0u);
-          rtab[c] = in ? t : kNone;
-          rs[c] = run_mask(rtab[c]);
-        }
-      }
-      // d/dψ_r: Σ over a segment's records of ā(r)[owner]·c·ℓ(γ1)ℓ(γ2), one
-      // atomic per run of records (records past their segment's rows have
-      // ψ_r·ā = 0 and add nothing)
-      auto add_psi = [&](uint32_t key, int r, Real x) {
-        if (TabL) lds_add(&accPS[AL.psi + key + r], x);
-        else acc_add<false>(W.acc + AL.psi + key + r, (double)x);
-      };
-      // Per row r (top down) two item passes and one record pass:
-      //   A(r)    items: ℓ of row r-1 (prefetched), φ_r·ā(r) into the other
-      //           buffer (the start of ā(r-1)) and ψ_r·ā(r) for the records
-      //   REC(r)  records: the ψ part scattered into the two parts' ā(r-1),
-      //           and the owner's split sum (forcing of d/dψ_r)
-      //   C(r)    items: d/dφ_r, d/dψ_r per segment
-      // C(r) and A(r-1) share one pass over the items.
-      // A(x) for item i from ā(x) = g into dst; pv = ℓ(x-1)
-      // (branch-free: past its segment's rows an item passes ā through and
-      // its records' forcing is zero; pv is then a finite stale value)
-      auto item_a = [&](const int x, const ItemRow& it, uint32_t i, Real g, Real pv, Real* dst) {
-        const bool in = x < (int)it.R;
-        Real ph, ps;
-        slice<Real, TabL>(M, sTab, in ? it.tab + x : 0u, ph, ps);
-        bV[i] = pv;
-        dst[i] = in ? ph * g : g;
-        sT[i] = in ? ps * g : Real(0);
-      };
-      auto refill = [&](const int x, Real (&pre)[kCacheB]) {  // ℓ(x-3) for A(x-2)
-#pragma unroll
-        for (int c = 0; c < kCacheB; ++c) {
-          const uint32_t i = lane + c * kWG;
-          if (i < n && x >= 3 && x - 2 < (int)cache[c].R) pre[c] = ell[cache[c].cell + (size_t)(x - 3) * cache[c].stride];
-        }
-      };
-      // C(r) for item i: d/dφ_r from ā(r) = g and ℓ(r-1) = v (all lanes take part)
-      auto item_c = [&](const int r, const ItemRow& it, bool on, Real g, Real v, const RunMask& rmk) {
-        Real cphi = on ? g * v : Real(0);
-        cphi = run_sum(cphi, rmk);
-        if (rmk.ends() && on && cphi != Real(0)) {
-          const uint32_t key = it.tab;
-          if (TabL) lds_add(&accPS[AL.phi + key + r], cphi);
-          else acc_add<false>(W.acc + AL.phi + key + r, (double)cphi);
-        }
-      };
-      // A(rtop) on its own
-      {
-#pragma unroll
-        for (int c = 0; c < kCacheB; ++c) {
-          const uint32_t i = lane + c * kWG;
-          if (i < n) item_a(rtop, cache[c], i, bA[i], preA[c], bB);
-        }
-        for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {  // uniform: lane shuffles
-          const uint32_t i = base + lane;
-          const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
-          if (i < n)
-            item_a(rtop, it, i, bA[i], rtop < (int)it.R ? ell[it.cell + (size_t)(rtop - 1) * it.stride] : Real(0), bB);
-        }
-        refill(rtop, preA);
-      }
-      auto rec_pass = [&](const int r) {
-        // records, one per lane; the cached records' reads are all issued
-        // before any is used
-        {
-          Real va[RC], vb[RC], g[RC], ga[RC];
-#pragma unroll
-          for (int c = 0; c < RC; ++c) {
-            if ((uint32_t)c * kWG < nrec) {
-              const uint32_t o = rown[c] != kNone ? rown[c] : 0u;
-              va[c] = bV[rops[c] & 0xffff];
-              vb[c] = bV[rops[c] >> 16];
-              g[c] = rown[c] != kNone ? sT[o] : Real(0);
-              ga[c] = bA[o];
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < RC; ++c) {
-            if ((uint32_t)c * kWG < nrec) {  // uniform: the run sums use DPP
-              const Real t = g[c] * rcoef[c];
-              if (t != Real(0)) {
-                lds_add(&bB[rops[c] & 0xffff], t * vb[c]);
-                lds_add(&bB[rops[c] >> 16], t * va[c]);
-              }
-              const Real x = run_sum(t != Real(0) ? ga[c] * rcoef[c] * va[c] * vb[c] : Real(0), rs[c]);
-              if (rs[c].ends() && x != Real(0)) add_psi(rtab[c], r, x);
-            }
-          }
-        }
-        for (uint32_t base = RC * kWG; base < nrec; base += kWG) {  // uniform: shuffles, row sums
-          const uint32_t k = base + lane;
-          const bool in = k < nrec;
-          const uint32_t own = in ? (uint32_t)sOwn[k] : 0u;
-          const uint32_t tb = ST.tab(B, M, sSegL[own]);
-          const uint32_t key = in ? tb : kNone;
-          Real x = 0;
-          if (in) {
-            const uint32_t ops = sOps[k];
-            const uint32_t a = ops & 0xffff, b = ops >> 16;
-            const Real c = sCoef[k];
-            const Real va = bV[a], vb = bV[b];
-            const Real t = sT[own] * c;
-            if (t != Real(0)) {
-              lds_add(&bB[a], t * vb);
-              lds_add(&bB[b], t * va);
-              x = bA[own] * c * va * vb;
-            }
-          }
-          bool last;
-          x = row_seg_sum(x, key, &last);
-          if (last && key != kNone && x != Real(0)) add_psi(key, r, x);
-        }
-      };
-      // REC(r), then C(r) with A(r-1); A(r-1) reads ℓ(r-2) from `pre`
-      auto row_step = [&](const int r, Real (&pre)[kCacheB]) {
-        wsync<Gws>();
-        rec_pass(r);
-        wsync<Gws>();
-        const bool more = r >= 2;  // uniform
-#pragma unroll
-        for (int c = 0; c < kCacheB; ++c) {
-          if ((uint32_t)c * kWG < n) {  // uniform: the run sums use DPP
-            const uint32_t i = lane + c * kWG;
-            const uint32_t ii = i < n ? i : 0u;
-            const Real g = bA[ii], v = bV[ii], g1 = bB[ii];
-            item_c(r, cache[c], i < n && r < (int)cache[c].R, g, v, im[c]);
-            if (more && i < n) item_a(r - 1, cache[c], i, g1, pre[c], bA);
-          }
-        }
-        for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {  // uniform: lane shuffles, DPP
-          const uint32_t i = base + lane;
-          const uint32_t ii = i < n ? i : 0u;
-          const ItemRow it = ST.row(B, M, i < n ? sSegL[i] : 0u, i);
-          const Real g = bA[ii], v = bV[ii], g1 = bB[ii];
-          item_c(r, it, i < n && r < (int)it.R, g, v, run_mask(i < n ? it.tab : kNone));
-          if (more && i < n)
-            item_a(r - 1, it, i, g1, r - 1 < (int)it.R ? ell[it.cell + (size_t)(r - 2) * it.stride] : Real(0), bA);
-        }
-        if (more) refill(r - 1, pre);
-        Real* t = bA;
-        bA = bB;
-        bB = t;
-      };
-      for (int r = rtop; r >= 1; r -= 2) {
-        row_step(r, preB);
-        if (r >= 2) row_step(r - 1, preA);
-      }
-      wsync<Gws>();
-      if (TabL) {
-        // move this stage's d/dφ_r, d/dψ_r into the global FP64 sums (each LDS
-        // partial then holds at most one stage's contributions per family)
-        for (uint32_t sl = 0; sl < S.n_segs; ++sl) {
-          uint32_t tab, R;
-          if (ST.inreg) {
-            tab = (uint32_t)__builtin_amdgcn_readlane((int)ST.mine.tab, (int)sl);
-            R = (uint32_t)__builtin_amdgcn_readlane((int)ST.mine.R, (int)sl);
-          } else {
-            const SegInfo g = seg_info(B, M, S.seg0 + sl);
-            tab = g.tab;
-            R = g.R;
-          }
-          for (uint32_t r = 1 + lane; r < R; r += kWG) {
-            const Real a = __hip_atomic_exchange(&accPS[AL.phi + tab + r], Real(0), __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-            const Real b = __hip_atomic_exchange(&accPS[AL.psi + tab + r], Real(0), __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (a != Real(0)) unsafeAtomicAdd(W.acc + AL.phi + tab + r, (double)a);
-            if (b != Real(0)) unsafeAtomicAdd(W.acc + AL.psi + tab + r, (double)b);
-          }
-        }
-      }
-      // row 0: adjoints of the child tops, ε and q
-      // pass over all items: cached tops for the first kCacheB per lane
-      auto each = [&](auto&& fn) {
-#pragma unroll
-        for (int c = 0; c < kCacheB; ++c) {
-          const uint32_t i = lane + c * kWG;
-          if (i < n) fn(tops[c], i);
-        }
-        for (uint32_t base = kCacheB * kWG; base < n; base += kWG) {
-          const uint32_t i = base + lane;
-          const SegInfo sg = ST.get(B, M, i < n ? sSegL[i] : 0u);
-          if (i < n) fn(item_top(sg, i), i);
-        }
-      };
-      // pass over all records: the first RC per lane from registers
-      auto each_rec = [&](auto&& fn) {
-#pragma unroll
-        for (int c = 0; c < RC; ++c)
-          if ((uint32_t)c * kWG < nrec && rown[c] != kNone) fn(rops[c], rown[c], rcoef[c]);
-        for (uint32_t k = RC * kWG + lane; k < nrec; k += kWG) fn(sOps[k], (uint32_t)sOwn[k], sCoef[k]);
-      };
-      // child tops; a WGD item's one child also stands in the child1 slot, so
-      // the split terms below read the "other" top from bG for both kinds
-      bool any_wgd = false;
-      each([&](const ItemTop& tp, uint32_t i) {
-        const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
-        const Real v0 = cf >= 0 ? ell[cf] : Real(0);
-        bV[i] = v0;
-        bG[i] = tp.kind == WHALE_NODE_WGD ? v0 : (cg >= 0 ? ell[cg] : Real(0));
-        sFo[i] = 0;
-        any_wgd |= tp.kind == WHALE_NODE_WGD;
-      });
-      wsync<Gws>();
-      if (__any(any_wgd)) {
-        // WGD retention sums Σ c·ℓ(γ1)ℓ(γ2) over the child's tops, per owner
-        each_rec([&](uint32_t ops, uint32_t own, Real c) {
-          const Real t = c * bV[ops & 0xffff] * bV[ops >> 16];
-          if (t != Real(0)) lds_add(&sFo[own], t);
-        });
-        wsync<Gws>();
-      }
-      // ε̄, q̄ and the direct terms of the child0 slot; then each owner's
-      // factor of its split terms in that slot
-      each([&](const ItemTop& tp, uint32_t i) {
-        const Real g = bA[i];
-        Real fac = 0;
-        if (tp.kind == WHALE_NODE_SPECIATION) {
-          const Real eg = NC.etop[tp.c1];
-          acc_add<TabL>(accX + AL.eps + tp.c1, (double)(g * bV[i]));
-          acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * bG[i]));
-          bB[i] = g * eg;
-          fac = g;
-        } else if (tp.kind == WHALE_NODE_WGD) {
-          const Real qq = NC.q[tp.wgd];
-          const Real ef = NC.etop[tp.c0];
-          const Real ret = sFo[i];
-          acc_add<TabL>(accX + AL.q + tp.wgd, (double)(g * (ret - bV[i] + Real(2) * ef * bV[i])));
-          acc_add<TabL>(accX + AL.eps + tp.c0, (double)(g * Real(2) * qq * bV[i]));
-          bB[i] = g * ((Real(1) - qq) + Real(2) * qq * ef);
-          fac = qq * g;
-        }
-        sFo[i] = fac;
-      });
-      wsync<Gws>();
-      // split terms of the child0 slot: speciation pairs child0 parts with child1
-      // tops of the other part; a WGD pairs both parts on its one child
-      each_rec([&](uint32_t ops, uint32_t own, Real c) {
-        const Real g = sFo[own] * c;
-        if (g == Real(0)) return;
-        const uint32_t a = ops & 0xffff, b = ops >> 16;
-        lds_add(&bB[a], g * bG[b]);
-        lds_add(&bB[b], g * bG[a]);
-      });
-      wsync<Gws>();
-      each([&](const ItemTop& tp, uint32_t i) {
-        if (tp.kind == WHALE_NODE_SPECIATION || tp.kind == WHALE_NODE_WGD) ibar[tp.ib] = bB[i];
-        const bool sp = tp.kind == WHALE_NODE_SPECIATION;
-        if (sp) bB[i] = bA[i] * NC.etop[tp.c0];
-        sFo[i] = sp ? bA[i] : Real(0);
-      });
-      wsync<Gws>();
-      // child1 slot of speciation nodes
-      each_rec([&](uint32_t ops, uint32_t own, Real c) {
-        const Real g = sFo[own] * c;
-        if (g == Real(0)) return;
-        const uint32_t a = ops & 0xffff, b = ops >> 16;
-        lds_add(&bB[a], g * bV[b]);
-        lds_add(&bB[b], g * bV[a]);
-      });
-      wsync<Gws>();
-      each([&](const ItemTop& tp, uint32_t i) {
-        if (tp.kind == WHALE_NODE_SPECIATION) ibar[tp.ib + tp.nc] = bB[i];
-      });
-    }
+    bwd_family<Real, Scaled, TabL, Gws>(B, M, W, L, f, wb, sTab, accPS, accX, NC, lane, W.ell + B.fam[f].ell0,
+                                        W.ibar + B.fam[f].ibar0, coefs);
   }
   if (TabL) {
     __syncthreads();
