@@ -192,6 +192,23 @@ int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t
 int whale_batch_set_tuning(whale_batch* b, const int64_t* in7);
 int whale_batch_get_tuning(const whale_batch* b, int64_t* out7);
 
+/* Memory budget (bytes) for the stored rows of ℓ, with ī and the record
+ * coefficients of the reverse sweep; 0 (the default) removes it.  When the
+ * whole batch's rows exceed it, evaluations that do not keep ℓ for the caller
+ * (whale_logpdf without WHALE_KEEP_ELL, whale_logpdf_grad) run each family's
+ * forward DP and reverse sweep in one pass of one wave, whose slot holds that
+ * family's rows only while the wave works on it: the rows take (resident
+ * waves) x (largest family) instead of growing with the batch, the launches'
+ * grids shrinking to fit the budget (one workgroup per launch at the least).
+ * Results equal the stored form's.  Calls that keep ℓ (logpdf!, download,
+ * backtracking) still store every row.  The reference keeps every ℓ in its
+ * CCDs (src/ccd.jl:86); no counterpart. */
+int whale_batch_set_ell_budget(whale_batch* b, int64_t bytes);
+/* [bytes the stored form needs for the whole batch, bytes of the slots the
+ * budgeted form uses (largest planned mode; 0 before its first call), bytes
+ * of both the batch holds now]. */
+int whale_batch_ell_bytes(const whale_batch* b, int64_t* out3);
+
 /* logpdf of every family (unconditioned, src/core.jl:31-43) into
  * out_family[n_families] (may be NULL) and the conditioned batch value
  * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */

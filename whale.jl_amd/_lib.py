@@ -68,6 +68,8 @@ def lib():
         L.whale_batch_set_lds_limit.argtypes = [vp, C.c_int64]
         L.whale_batch_autotune.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_char_p, i64p, dblp]
         L.whale_batch_set_tuning.argtypes = [vp, i64p]
+        L.whale_batch_set_ell_budget.argtypes = [vp, C.c_int64]
+        L.whale_batch_ell_bytes.argtypes = [vp, i64p]
         L.whale_batch_get_tuning.argtypes = [vp, i64p]
         L.whale_batch_launches.argtypes = [vp, C.c_int32, i64p]
         L.whale_logpdf.argtypes = [vp, C.c_int32, C.c_int32, dblp, dblp]

@@ -198,6 +198,16 @@ class DeviceBatch:
         return dict(class_bounds=[int(x) for x in o[:5]], layout=int(o[5]), waves_per_cu=int(o[6]),
                     candidates_timed=int(o[7]), ms_before=float(ms[0]), ms_after=float(ms[1]))
 
+    def set_ell_budget(self, nbytes: int):
+        """Bounds the memory of stored ℓ rows (0: none); above it evaluations run
+        each family's forward and reverse sweep in one pass (whale_batch_set_ell_budget)."""
+        check(_lib.lib().whale_batch_set_ell_budget(self.h, int(nbytes)), "whale_batch_set_ell_budget")
+
+    def ell_bytes(self) -> dict:
+        o = np.zeros(3, np.int64)
+        check(_lib.lib().whale_batch_ell_bytes(self.h, ptr(o, i64p)), "whale_batch_ell_bytes")
+        return dict(stored_form=int(o[0]), budgeted_slots=int(o[1]), held=int(o[2]))
+
     def tuning(self) -> list:
         o = np.zeros(7, np.int64)
         check(_lib.lib().whale_batch_get_tuning(self.h, ptr(o, i64p)), "whale_batch_get_tuning")

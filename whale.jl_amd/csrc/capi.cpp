@@ -49,6 +49,11 @@ struct DBuf {
     if (e == hipSuccess) n = bytes;
     return e;
   }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
@@ -95,6 +100,7 @@ int assign_classes(whale_batch* b);
 int plan_batch_launches(whale_batch* b);
 int ensure_gws(whale_batch* b, bool* moved);
 void drop_graphs(whale_batch* b);
+uint64_t full_bytes(const whale_batch* b);
 
 // LDS one wave needs for family d: its largest non-root stage (items and
 // records bounded separately) and its root stage, root records staged or not
@@ -189,6 +195,18 @@ struct whale_batch {
   // bounds of the size classes, how the classes are spread over the three
   // streams, and the most waves per compute unit a launch plans for.
   Tuning tune;
+  // Memory budget for the stored rows of ℓ (0: none).  When the whole
+  // batch's ℓ, ī and coefficients exceed it, evaluations without stored ℓ
+  // for the caller run the fused forward + reverse launches, whose per-wave
+  // slots hold one family at a time (k_fused); their launch shapes and slots,
+  // per mode, are planned on first use.
+  size_t ell_budget = 0;
+  bool fused_ok[3] = {};
+  LaunchCfg fcfg[3][kClasses], fcfg_hbm[3], fcfg_route[3], fcfg_fb;
+  FusedSlots fslot[3][kClasses], fslot_hbm[3], fslot_route[3], fslot_fb[3];
+  size_t fused_bytes[3] = {};
+  DBuf fslots;
+  bool full_ok = false;  // ell, coef and ibar allocated for every family
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   int last_mode = -1;
@@ -205,11 +223,11 @@ struct whale_batch {
   LaunchCfg cfg_route[3][2];
   std::vector<uint32_t> order_host;
   // captured evaluations, keyed by (mode, gradient, keep ℓ), and pinned result staging
-  hipGraphExec_t graph[3][2][2] = {};
+  hipGraphExec_t graph[3][2][2][2] = {};  // [mode][gradient][keep ℓ][fused]
   double* pin_out = nullptr;
   double* pin_fam = nullptr;
   ~whale_batch() {
-    for (auto& a : graph) for (auto& b2 : a) for (auto& g : b2) if (g) (void)hipGraphExecDestroy(g);
+    for (auto& a : graph) for (auto& b2 : a) for (auto& c2 : b2) for (auto& g : c2) if (g) (void)hipGraphExecDestroy(g);
     if (pin_out) (void)hipHostFree(pin_out);
     if (pin_fam) (void)hipHostFree(pin_fam);
   }
@@ -660,10 +678,9 @@ int whale_batch_create(whale_model* m, int32_t n_fam, const int64_t* fam_clade_o
     HIPCHK(upload(b->spl_g, sg, st));
   }
   b->n_rec = P.rec_ops.size();
-  HIPCHK(b->ell.alloc(16 + 12 * std::max<uint64_t>(1, P.ell_cells)));
-  HIPCHK(b->coef.alloc(16 + 12 * std::max<uint64_t>(1, b->n_rec)));
+
   HIPCHK(b->E.alloc(sizeof(int32_t) * std::max<int64_t>(1, nc)));
-  HIPCHK(b->ibar.alloc(16 + 12 * std::max<uint64_t>(1, P.ibar_cells)));
+
   HIPCHK(b->flag.alloc(sizeof(uint32_t) * n_fam));
   HIPCHK(b->fb.alloc(sizeof(uint32_t) * (n_fam + 1)));
   HIPCHK(b->fam_ll.alloc(sizeof(double) * n_fam));
@@ -734,6 +751,35 @@ int whale_batch_set_lds_limit(whale_batch* b, int64_t bytes) {
   if ((rc = ensure_gws(b, nullptr)) != WHALE_OK) return rc;
   HIPCHK(hipMemcpy(b->order.p, b->order_host.data(), sizeof(uint32_t) * b->n_fam, hipMemcpyHostToDevice));
   drop_graphs(b);
+  return WHALE_OK;
+}
+
+int whale_batch_set_ell_budget(whale_batch* b, int64_t bytes) {
+  if (!b) return fail(WHALE_EINVAL, "NULL argument");
+  if (bytes < 0) return fail(WHALE_EINVAL, "negative budget");
+  HIPCHK(hipSetDevice(b->device));
+  b->ell_budget = (size_t)bytes;
+  for (bool& x : b->fused_ok) x = false;
+  drop_graphs(b);
+  if (b->ell_budget && full_bytes(b) > b->ell_budget && b->full_ok) {
+    // the stored form's buffers are given back; a call that keeps ℓ allocates them again
+    b->ell.free();
+    b->coef.free();
+    b->ibar.free();
+    b->full_ok = false;
+    if (b->last_store_all) b->last_mode = -1;  // nothing stored any more
+  }
+  return WHALE_OK;
+}
+
+int whale_batch_ell_bytes(const whale_batch* b, int64_t* out3) {
+  if (!b || !out3) return fail(WHALE_EINVAL, "NULL argument");
+  out3[0] = (int64_t)full_bytes(b);
+  size_t fz = 0;
+  for (int m = 0; m < 3; ++m)
+    if (b->fused_ok[m]) fz = std::max(fz, b->fused_bytes[m]);
+  out3[1] = (int64_t)fz;
+  out3[2] = (int64_t)((b->full_ok ? b->ell.n + b->coef.n + b->ibar.n : 0) + b->fslots.n);
   return WHALE_OK;
 }
 
@@ -911,6 +957,7 @@ int assign_classes(whale_batch* b) {
   }
   b->route_lo = (uint32_t)b->order_host.size();
   b->order_host.insert(b->order_host.end(), rt.begin(), rt.end());
+  for (bool& x : b->fused_ok) x = false;  // fused shapes follow the classes
   int rc = plan_batch_launches(b);
   if (rc != WHALE_OK) return rc;
   for (int mode = 0; mode < 3; ++mode)
@@ -934,6 +981,11 @@ int ensure_gws(whale_batch* b, bool* moved) {
       if (b->route_lo < b->n_fam) route = std::max(route, b->cfg_route[mode][dir].gws_bytes());
     }
   for (int dir = 0; dir < 2; ++dir) main = std::max(main, b->cfg_fb[dir].gws_bytes());
+  for (int mode = 0; mode < 3; ++mode)
+    if (b->fused_ok[mode]) {
+      main = std::max(main, std::max(b->fcfg_hbm[mode].gws_bytes(), b->fcfg_fb.gws_bytes()));
+      if (b->route_lo < b->n_fam) route = std::max(route, b->fcfg_route[mode].gws_bytes());
+    }
   for (uint32_t k = b->hbm_lo; k < b->hbm_hi; ++k)
     for (int dir = 0; dir < 2; ++dir)
       main = std::max(main, align16(fam_lds(b->plan.fam[b->order_host[k]], WHALE_FP64_SCALED, dir == 0, false)));
@@ -949,11 +1001,153 @@ int ensure_gws(whale_batch* b, bool* moved) {
 void drop_graphs(whale_batch* b) {
   for (auto& a : b->graph)
     for (auto& b2 : a)
-      for (auto& g : b2)
-        if (g) {
-          (void)hipGraphExecDestroy(g);
-          g = nullptr;
-        }
+      for (auto& c2 : b2)
+        for (auto& g : c2)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
+}
+
+// ℓ, coefficients and ī of every family (8 bytes per value, or 4 with the
+// FP64 region of recomputed families after it in the FP32 form)
+uint64_t full_bytes(const whale_batch* b) {
+  const PlanHost& P = b->plan;
+  return (16 + 12 * std::max<uint64_t>(1, P.ell_cells)) + (16 + 12 * std::max<uint64_t>(1, b->n_rec)) +
+         (16 + 12 * std::max<uint64_t>(1, P.ibar_cells));
+}
+
+int ensure_full(whale_batch* b) {
+  if (b->full_ok) return WHALE_OK;
+  const PlanHost& P = b->plan;
+  HIPCHK(b->ell.alloc(16 + 12 * std::max<uint64_t>(1, P.ell_cells)));
+  HIPCHK(b->coef.alloc(16 + 12 * std::max<uint64_t>(1, b->n_rec)));
+  HIPCHK(b->ibar.alloc(16 + 12 * std::max<uint64_t>(1, P.ibar_cells)));
+  b->full_ok = true;
+  return WHALE_OK;
+}
+
+// launch shape of a fused launch over `fams`: the wave's stage region fits
+// both directions, the shared LDS is the reverse sweep's
+bool plan_fused_launch(const whale_batch* b, int mode, const uint32_t* fams, size_t nf, uint64_t grid_fams, bool multi,
+                       bool force_gws, LaunchCfg* out) {
+  size_t with = 16, without = 16;
+  for (size_t j = 0; j < nf; ++j) {
+    const FamDev& d = b->plan.fam[fams[j]];
+    with = std::max({with, fam_lds(d, mode, true, true), fam_lds(d, mode, false, true)});
+    without = std::max({without, fam_lds(d, mode, true, false), fam_lds(d, mode, false, false)});
+  }
+  const whale_model* m = b->model;
+  LaunchCfg a, c;
+  const bool oa = !force_gws && plan_launch(m, false, mode, with, grid_fams, multi, &a, b->tune.waves_cap);
+  const bool oc = !force_gws && plan_launch(m, false, mode, without, grid_fams, multi, &c, b->tune.waves_cap);
+  if (oa && (!oc || a.waves_per_cu >= c.waves_per_cu)) *out = a;
+  else if (oc) *out = c;
+  if (oa || oc) return true;
+  const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+  LaunchCfg g;
+  g.gws = true;
+  g.fpw = 1;
+  g.tabl = false;
+  g.waves_per_cu = 1;
+  g.L.wave_bytes = (uint32_t)align16(without);
+  g.L.shared_bytes = (uint32_t)bwd_shared_bytes(mode, false, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd);
+  if (g.L.shared_bytes > m->ctx->lds_wg || without > 0xffffffffull) return false;
+  g.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)m->ctx->n_cu, grid_fams));
+  *out = g;
+  return true;
+}
+
+// a slot of a fused launch over `fams`: the largest family's rows, ī and coefficients
+void fused_slot(const whale_batch* b, int mode, const uint32_t* fams, size_t nf, FusedSlots* out) {
+  const PlanHost& P = b->plan;
+  const uint64_t s = mode == WHALE_FP32_SCALED ? 4 : 8;
+  uint64_t cells = 1, ibar = 1, recs = 1;
+  for (size_t j = 0; j < nf; ++j) {
+    const uint32_t f = fams[j];
+    const FamDev& d = P.fam[f];
+    const bool last = f + 1 == b->n_fam;
+    cells = std::max(cells, (last ? P.ell_cells : P.fam[f + 1].ell0) - d.ell0);
+    ibar = std::max(ibar, (last ? P.ibar_cells : P.fam[f + 1].ibar0) - d.ibar0);
+    recs = std::max<uint64_t>(recs, d.n_recs);
+  }
+  auto al = [](uint64_t x) { return (x + 255) & ~(uint64_t)255; };
+  FusedSlots S;
+  S.ibar_off = al(cells * s);
+  S.coef_off = S.ibar_off + al(ibar * s);
+  S.slot_bytes = S.coef_off + al(recs * s);
+  *out = S;
+}
+
+// Shapes and slots of every fused launch of `mode` (classes, families in
+// global memory, routed families, the FP64 pass over flagged ones), their
+// grids shrunk so that all slots fit the budget (at least one workgroup per
+// launch), and the slot buffer.
+int plan_fused(whale_batch* b, int mode) {
+  const whale_model* m = b->model;
+  const uint32_t* ord = b->order_host.data();
+  const int smode = mode == WHALE_FP32_SCALED ? WHALE_FP64_SCALED : mode;  // routed and flagged families
+  struct Item { LaunchCfg* c; FusedSlots* s; };
+  std::vector<Item> items;
+  for (int k = 0; k < whale_batch::kClasses; ++k) {
+    const uint32_t lo = b->cls_lo[k], hi = b->cls_hi[k];
+    if (hi == lo) continue;
+    if (!plan_fused_launch(b, mode, ord + lo, hi - lo, hi - lo, true, false, &b->fcfg[mode][k]))
+      return fail(WHALE_EINVAL, "no fused launch shape");
+    fused_slot(b, mode, ord + lo, hi - lo, &b->fslot[mode][k]);
+    items.push_back({&b->fcfg[mode][k], &b->fslot[mode][k]});
+  }
+  if (b->hbm_hi > b->hbm_lo) {
+    const uint32_t nh = b->hbm_hi - b->hbm_lo;
+    if (!plan_fused_launch(b, mode, ord + b->hbm_lo, nh, nh, false, true, &b->fcfg_hbm[mode]))
+      return fail(WHALE_EINVAL, "no fused launch shape");
+    fused_slot(b, mode, ord + b->hbm_lo, nh, &b->fslot_hbm[mode]);
+    items.push_back({&b->fcfg_hbm[mode], &b->fslot_hbm[mode]});
+  }
+  if (b->route_lo < b->n_fam) {
+    const uint32_t nr = b->n_fam - b->route_lo;
+    if (!plan_fused_launch(b, smode, ord + b->route_lo, nr, nr, true, false, &b->fcfg_route[mode]))
+      return fail(WHALE_EINVAL, "no fused launch shape");
+    fused_slot(b, smode, ord + b->route_lo, nr, &b->fslot_route[mode]);
+    items.push_back({&b->fcfg_route[mode], &b->fslot_route[mode]});
+  }
+  if (mode == WHALE_FP32_SCALED) {
+    if (!plan_fused_launch(b, WHALE_FP64_SCALED, ord, b->n_fam, (uint64_t)m->ctx->n_cu, false, false, &b->fcfg_fb))
+      return fail(WHALE_EINVAL, "no fused launch shape");
+    fused_slot(b, WHALE_FP64_SCALED, ord, b->n_fam, &b->fslot_fb[mode]);
+    items.push_back({&b->fcfg_fb, &b->fslot_fb[mode]});
+  }
+  auto total = [&]() {
+    uint64_t t = 0;
+    for (auto& it : items) t += (uint64_t)it.c->grid * it.c->fpw * it.s->slot_bytes;
+    return t;
+  };
+  if (b->ell_budget && total() > b->ell_budget) {
+    const double f = (double)b->ell_budget / (double)total();
+    for (auto& it : items) it.c->grid = std::max(1, (int)((double)it.c->grid * f));
+  }
+  uint64_t off = 0;
+  for (auto& it : items) off += (uint64_t)it.c->grid * it.c->fpw * it.s->slot_bytes;
+  const void* before = b->fslots.p;
+  HIPCHK(b->fslots.alloc(std::max<uint64_t>(off, 256)));
+  if (b->fslots.p != before)
+    for (bool& x : b->fused_ok) x = false;  // every mode's slots start at the buffer; one mode runs at a time
+  off = 0;
+  for (auto& it : items) {
+    it.s->base = b->fslots.as<unsigned char>() + off;
+    off += (uint64_t)it.c->grid * it.c->fpw * it.s->slot_bytes;
+  }
+  b->fused_bytes[mode] = off;
+  b->fused_ok[mode] = true;
+  bool moved = false;
+  const int rc = ensure_gws(b, &moved);
+  if (rc != WHALE_OK) return rc;
+  drop_graphs(b);  // captured evaluations hold the previous slots
+  return WHALE_OK;
+}
+
+bool use_fused(const whale_batch* b, int32_t launch, bool keep) {
+  return b->ell_budget && launch == WHALE_LAUNCH_BATCHED && !keep && full_bytes(b) > b->ell_budget;
 }
 
 // All device work of one evaluation on the context's stream (capturable: no
@@ -961,7 +1155,7 @@ void drop_graphs(whale_batch* b) {
 // `ws`, when given, replaces the batch's workspace (and `ws_out` the model's
 // result buffer); the results then stay on the device.
 int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all, bool timed,
-            const WorkRaw* ws = nullptr, double* ws_out = nullptr) {
+            const WorkRaw* ws = nullptr, double* ws_out = nullptr, bool fused = false) {
   whale_model* m = b->model;
   whale_ctx* c = m->ctx;
   hipStream_t st = c->stream;
@@ -1081,6 +1275,59 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
     return WHALE_OK;
   };
   int rc;
+  if (fused) {
+    // every family's forward and reverse sweep by one wave, its rows in the
+    // wave's slot (k_fused); the same streams and classes as the stored form
+    const int rmode = fp32 ? WHALE_FP64_SCALED : mode;
+    const bool routed = b->route_lo < nf;
+    if (routed) {
+      HIPCHK(hipEventRecord(c->fork[6], st));
+      HIPCHK(hipStreamWaitEvent(c->side2, c->fork[6], 0));
+      WorkRaw Wr = W;
+      if (Wr.gws) Wr.gws += b->gws_main;
+      Wr.ticket = W.ticket + 2 * whale_batch::kClasses + 2;
+      HIPCHK(launch_fused(rmode, b->fcfg_route[mode], c->side2, B, M, Wr, b->route_lo, nf, nullptr, grad ? 1 : 0,
+                          b->fslot_route[mode]));
+      HIPCHK(hipEventRecord(c->fork[7], c->side2));
+    }
+    constexpr int kLanes = 3;
+    hipStream_t lanes[kLanes] = {st, c->side, c->side3};
+    const int* kOrder = kLayoutOrder[b->tune.layout];
+    const int* kLane = kLayoutLane[b->tune.layout];
+    bool used[kLanes] = {};
+    for (int k = 0; k < whale_batch::kClasses; ++k)
+      if (b->cls_hi[k] > b->cls_lo[k]) used[kLane[k]] = true;
+    HIPCHK(hipEventRecord(c->fork[0], st));
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], c->fork[0], 0));
+    if (b->hbm_hi > b->hbm_lo) {
+      WorkRaw Wh = W;
+      Wh.ticket = W.ticket + kTicketHbm;
+      HIPCHK(launch_fused(mode, b->fcfg_hbm[mode], st, B, M, Wh, b->hbm_lo, b->hbm_hi, nullptr, grad ? 1 : 0,
+                          b->fslot_hbm[mode]));
+    }
+    for (int j = 0; j < whale_batch::kClasses; ++j) {
+      const int k = kOrder[j];
+      if (b->cls_hi[k] == b->cls_lo[k]) continue;
+      WorkRaw Wk = W;
+      Wk.ticket = W.ticket + k;
+      HIPCHK(launch_fused(mode, b->fcfg[mode][k], lanes[kLane[k]], B, M, Wk, b->cls_lo[k], b->cls_hi[k], nullptr,
+                          grad ? 1 : 0, b->fslot[mode][k]));
+    }
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) {
+        HIPCHK(hipEventRecord(c->fork[3 + l], lanes[l]));
+        HIPCHK(hipStreamWaitEvent(st, c->fork[3 + l], 0));
+      }
+    if (fp32) {
+      Wfb.ticket = W.ticket + 2 * whale_batch::kClasses;
+      HIPCHK(launch_fused(WHALE_FP64_SCALED, b->fcfg_fb, st, Bfb, M, Wfb, 0, 0, W.fb, grad ? 1 : 0,
+                          b->fslot_fb[mode]));
+    }
+    if (timed) HIPCHK(hipEventRecord(c->ev[2], st));
+    if (routed) HIPCHK(hipStreamWaitEvent(st, c->fork[7], 0));
+    if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
+  } else {
   // families routed to FP64: their whole evaluation on the second side stream
   const bool routed = launch == WHALE_LAUNCH_BATCHED && b->route_lo < nf;
   if (routed) {
@@ -1129,6 +1376,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   }
   if (routed) HIPCHK(hipStreamWaitEvent(st, c->fork[7], 0));
   if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
+  }
   double* out = ws_out ? ws_out : m->out.as<double>();
   HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
                      m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out));
@@ -1163,12 +1411,22 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
     HIPCHK(hipMemcpy(b->order.p, b->order_host.data(), sizeof(uint32_t) * b->n_fam, hipMemcpyHostToDevice));
     drop_graphs(b);
   }
+  const bool fused = use_fused(b, launch, keep);
+  if (fused) {
+    if (!b->fused_ok[mode]) {
+      const int rc = plan_fused(b, mode);
+      if (rc != WHALE_OK) return rc;
+    }
+  } else {
+    const int rc = ensure_full(b);
+    if (rc != WHALE_OK) return rc;
+  }
   if (launch == WHALE_LAUNCH_BATCHED && !c->timing) {
     // replay the captured evaluation (captured on first use)
-    hipGraphExec_t& ge = b->graph[mode][grad ? 1 : 0][keep ? 1 : 0];
+    hipGraphExec_t& ge = b->graph[mode][grad ? 1 : 0][keep ? 1 : 0][fused ? 1 : 0];
     if (!ge) {
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
-      int rc = enqueue(b, mode, launch, grad, store_all, false);
+      int rc = enqueue(b, mode, launch, grad, store_all, false, nullptr, nullptr, fused);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(st, &g);
       if (rc != WHALE_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -1179,7 +1437,7 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
     }
     HIPCHK(hipGraphLaunch(ge, st));
   } else {
-    int rc = enqueue(b, mode, launch, grad, store_all, c->timing);
+    int rc = enqueue(b, mode, launch, grad, store_all, c->timing, nullptr, nullptr, fused);
     if (rc != WHALE_OK) return rc;
   }
   HIPCHK(hipStreamSynchronize(st));
@@ -1205,7 +1463,7 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
     if (d_eta) *d_eta = o[1 + 2 * n + nw];
   }
   b->last_mode = mode;
-  b->last_store_all = store_all;
+  b->last_store_all = store_all && !fused;
   b->last_launch = launch;
   b->last_fallback = 0;
   if (mode == WHALE_FP32_SCALED) {
@@ -1256,7 +1514,7 @@ int time_evals(whale_batch* b, int mode, bool grad, int reps, double* ms) {
 
 // The workload a tuning applies to: device, arithmetic, gradient or not, and
 // every family's shape as the planner sees it.
-std::string tune_key(const whale_batch* b, int mode, bool grad) {
+void tune_key(const whale_batch* b, int mode, bool grad, std::string* key) {
   const whale_model* m = b->model;
   uint64_t h = 1469598103934665603ull;
   auto mix = [&](uint64_t v) {
@@ -1283,7 +1541,7 @@ std::string tune_key(const whale_batch* b, int mode, bool grad) {
   char buf[256];
   snprintf(buf, sizeof buf, "%s/cu%d/lds%zu/mode%d/grad%d/%016llx", arch.c_str(), m->ctx->n_cu, m->ctx->lds_wg, mode,
            grad ? 1 : 0, (unsigned long long)h);
-  return buf;
+  *key = buf;
 }
 
 bool valid_tuning(const Tuning& t) {
@@ -1336,7 +1594,8 @@ int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t
   reps = std::max(1, std::min(reps, 50));
   const bool grad = gradient != 0;
   HIPCHK(hipSetDevice(b->device));
-  const std::string key = tune_key(b, mode, grad);
+  std::string key;
+  tune_key(b, mode, grad, &key);
   const Tuning start = b->tune;
   int rc;
   double t_start = 0, t_best = 0;

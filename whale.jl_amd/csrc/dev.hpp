@@ -135,6 +135,18 @@ hipError_t launch_forward(int mode, bool store_all, const LaunchCfg& C, hipStrea
 hipError_t launch_backward(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
                            const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                            const uint32_t* fam_count = nullptr);
+// per-wave slots of the fused forward + reverse launch: each holds one
+// family's stored rows (from byte 0), ī (from ibar_off) and record
+// coefficients (from coef_off)
+struct FusedSlots {
+  unsigned char* base = nullptr;
+  uint64_t slot_bytes = 0;
+  uint64_t ibar_off = 0;
+  uint64_t coef_off = 0;
+};
+hipError_t launch_fused(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                        const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count, int grad,
+                        const FusedSlots& SL);
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
                          double* dnode, const int32_t* hlevels, const int32_t* hnodes, int n_levels);
 hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,

@@ -610,6 +610,8 @@ std::string build_plan(const ModelHost& m, int n_fam, const int64_t* fam_clade_o
     d.max_recs = F.max_recs;
     d.root_recs = F.root_recs;
     d.root_seg = seg_base + F.root_seg;
+    d.rec0 = rec_base;
+    d.n_recs = (uint32_t)F.rec_ops.size();
     d.ell0 = ell;
     d.ibar0 = ibar;
     P.fam_seg0[f] = seg_base;

@@ -555,7 +555,8 @@ constexpr int kMaxFPW = 8;  // families (waves) per workgroup, at most
 template <typename Real, bool Scaled, bool StoreAll, bool TabL, bool Gws>
 __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M, const WorkDev<Real>& W,
                                            const LdsCfg& L, uint32_t f, unsigned char* const wb, Real* sTab,
-                                           const NodeC<Real>& NC, uint32_t lane, Real* ell, Real* coef) {
+                                           const NodeC<Real>& NC, uint32_t lane, Real* ell, Real* coef,
+                                           uint32_t coef_base) {
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   const FamDev F = B.fam[f];
   int32_t* E = W.E + F.clade0;
@@ -802,7 +803,7 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
         const Real p = sP[k];
         const Real c = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
         sCoef[k] = Eg == kUndef ? p : c;
-        if (StoreAll) coef[rb + k] = c;
+        if (StoreAll) coef[rb + k - coef_base] = c;
       }
     };
     // items still without values (open clades with splits): they take the
@@ -1158,7 +1159,7 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
         const uint32_t ops = inl ? sOps[k] : B.rec_ops[rb + k];
         const Real p = inl ? sCoef[k] : (Real)B.rec_p[rb + k];
         const int Eg = sEi[inl ? sOwn[k] : B.rec_own[rb + k]];
-        coef[rb + k] = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
+        coef[rb + k - coef_base] = coef_of(p, sEi[ops & 0xffff], sEi[ops >> 16], Eg);
       }
     }
   }
@@ -1202,7 +1203,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M
     }
     first = false;
     fwd_family<Real, Scaled, StoreAll, TabL, Gws>(B, M, W, L, f, wb, sTab, NC, lane, W.ell + B.fam[f].ell0,
-                                                  W.coef);
+                                                  W.coef, 0u);
   }
 }
 
@@ -1288,11 +1289,12 @@ template <typename Real, bool Scaled, bool TabL, bool Gws>
 __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M, const WorkDev<Real>& W,
                                            const LdsCfg& L, uint32_t f, unsigned char* const wb, Real* sTab,
                                            Real* const accPS, double* const accX, const NodeC<Real>& NC,
-                                           uint32_t lane, const Real* ell, Real* ibar, const Real* coefs) {
+                                           uint32_t lane, const Real* ell, Real* ibar, const Real* coefs,
+                                           uint32_t coef_base) {
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
   const FamDev F = B.fam[f];
-  auto coef_at = [&](uint32_t k) -> Real { return Scaled ? coefs[k] : (Real)B.rec_p[k]; };
+  auto coef_at = [&](uint32_t k) -> Real { return Scaled ? coefs[k - coef_base] : (Real)B.rec_p[k]; };
   // ---------------- root
   {
     const uint32_t n = F.n_clades;
@@ -1808,7 +1810,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
     first = false;
     if (Scaled && sizeof(Real) == sizeof(float) && W.flag[f]) continue;  // done by the FP64 pass
     bwd_family<Real, Scaled, TabL, Gws>(B, M, W, L, f, wb, sTab, accPS, accX, NC, lane, W.ell + B.fam[f].ell0,
-                                        W.ibar + B.fam[f].ibar0, coefs);
+                                        W.ibar + B.fam[f].ibar0, coefs, 0u);
   }
   if (TabL) {
     __syncthreads();
@@ -1826,6 +1828,73 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
 // ---------------------------------------------------------------------------
 // tail: one workgroup.  Batch sum, conditioning, table adjoint, chain rule.
 // out: [total, dλ(n), dμ(n), dq(n_wgd), dη]
+// ---------------------------------------------------------------------------
+// k_fused: the forward DP and (grad) the reverse sweep of each family by one
+// wave, with the family's stored rows, ī and record coefficients in the
+// wave's own slot of a scratch buffer.  A family's rows live only while its
+// wave works on it, so ℓ takes (resident waves) x (the launch's largest
+// family) instead of every cell of the batch: the evaluation under a memory
+// budget for ℓ.  Shared LDS as in the reverse sweep; a wave's stage region
+// fits both directions.
+// ---------------------------------------------------------------------------
+template <typename Real, bool Scaled, bool TabL, bool Gws>
+__global__ __launch_bounds__(kWG* kMaxFPW) void k_fused(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+                                                     uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count,
+                                                     int grad, FusedSlots SL) {
+  if (fam_count) fam_end = fam_begin + *fam_count;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t lane = threadIdx.x % kWG;
+  const uint32_t wv = threadIdx.x / kWG;
+  const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
+  const int T2 = 2 * M.tab_rows;
+  Real* sTab = reinterpret_cast<Real*>(smem);
+  const size_t tab_b = align16((size_t)T2 * sizeof(Real));
+  Real* const accPS = reinterpret_cast<Real*>(smem + tab_b);
+  double* const accX = TabL ? reinterpret_cast<double*>(smem + 2 * tab_b) - AL.eps : W.acc;
+  NodeC<Real> NC(smem + (TabL ? 2 * tab_b + align16(8 * (size_t)(AL.n - AL.eps)) : 0), M.n);
+  if (TabL) {
+    for (int r = threadIdx.x; r < M.tab_rows; r += (int)blockDim.x) {
+      sTab[2 * r] = (Real)M.phi[r];
+      sTab[2 * r + 1] = (Real)M.psi[r];
+    }
+    for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) accPS[j] = 0;
+    for (int j = AL.eps + threadIdx.x; j < AL.n; j += (int)blockDim.x) accX[j] = 0.0;
+  }
+  NC.fill(M, threadIdx.x, (int)blockDim.x);
+  __syncthreads();
+  const size_t slot = (size_t)blockIdx.x * (blockDim.x / kWG) + wv;
+  unsigned char* const wb = Gws ? W.gws + slot * L.wave_bytes : smem + L.shared_bytes + (size_t)wv * L.wave_bytes;
+  unsigned char* const sp = SL.base + slot * SL.slot_bytes;
+  Real* const ell = reinterpret_cast<Real*>(sp);
+  Real* const ibar = reinterpret_cast<Real*>(sp + SL.ibar_off);
+  Real* const coef = reinterpret_cast<Real*>(sp + SL.coef_off);
+  for (;;) {
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(W.ticket, 1u);
+    k = fam_begin + __builtin_amdgcn_readfirstlane(k);
+    if (k >= fam_end) break;
+    const uint32_t f = __builtin_amdgcn_readfirstlane(B.order[k]);
+    const uint32_t rec0 = B.fam[f].rec0;
+    wave_sync_mem();  // the slot's previous family is done with it
+    fwd_family<Real, Scaled, true, TabL, Gws>(B, M, W, L, f, wb, sTab, NC, lane, ell, coef, rec0);
+    if (!grad) continue;
+    if (Scaled && sizeof(Real) == sizeof(float) && W.flag[f]) continue;  // done by the FP64 pass
+    wave_sync_mem();  // the forward's rows and coefficients are visible to the wave
+    bwd_family<Real, Scaled, TabL, Gws>(B, M, W, L, f, wb, sTab, accPS, accX, NC, lane, ell, ibar, coef, rec0);
+  }
+  if (TabL && grad) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) {
+      const Real v = accPS[j];
+      if (v != Real(0)) unsafeAtomicAdd(W.acc + j, (double)v);
+    }
+    for (int j = AL.eps + threadIdx.x; j < AL.n; j += (int)blockDim.x) {
+      const double v = accX[j];
+      if (v != 0.0) unsafeAtomicAdd(W.acc + j, v);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 __global__ void k_tail(ModelDev M, const double* fam_ll, uint32_t n_fam, const double* acc_in, double* ebar,
                        const double* drow, const double* dnode, const int32_t* dlevels, const int32_t* dnodes,
@@ -2020,6 +2089,34 @@ hipError_t launch_backward(int mode, const LaunchCfg& C, hipStream_t st, const B
   if (mode == WHALE_FP64_PLAIN) return bwd_variant<double, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
   if (mode == WHALE_FP32_SCALED) return bwd_variant<float, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
   return bwd_variant<double, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+}
+
+template <typename R, bool S>
+static hipError_t fused_variant(const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                                const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count,
+                                int grad, const FusedSlots& SL) {
+  const size_t lds = C.lds();
+  const WorkDev<R> w(W);
+  if (C.fpw < 1 || C.fpw > kMaxFPW || !SL.base) return hipErrorInvalidValue;
+  if (C.gws) {
+    if (C.fpw != 1 || C.tabl || !W.gws) return hipErrorInvalidValue;
+    return go(k_fused<R, S, false, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_begin, fam_end, fam_count, grad, SL);
+  }
+  if (C.tabl)
+    return go(k_fused<R, S, true, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_begin, fam_end, fam_count, grad,
+              SL);
+  return go(k_fused<R, S, false, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_begin, fam_end, fam_count, grad,
+            SL);
+}
+
+hipError_t launch_fused(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
+                        const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count, int grad,
+                        const FusedSlots& SL) {
+  if (mode == WHALE_FP64_PLAIN)
+    return fused_variant<double, false>(C, st, B, M, W, fam_begin, fam_end, fam_count, grad, SL);
+  if (mode == WHALE_FP32_SCALED)
+    return fused_variant<float, true>(C, st, B, M, W, fam_begin, fam_end, fam_count, grad, SL);
+  return fused_variant<double, true>(C, st, B, M, W, fam_begin, fam_end, fam_count, grad, SL);
 }
 
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,

@@ -54,6 +54,8 @@ struct FamDev {
   uint32_t max_recs;     // most records of any non-root stage
   uint32_t root_seg;     // absolute segment of the root
   uint32_t root_recs;    // records of the root stage (all splits of the family)
+  uint32_t rec0;         // first record (absolute); the family's records are contiguous
+  uint32_t n_recs;       // records of all its stages and its root
   uint64_t ell0;         // first ℓ cell (absolute)
   uint64_t ibar0;        // first ī cell (absolute)
 };
