@@ -209,6 +209,17 @@ int whale_batch_set_ell_budget(whale_batch* b, int64_t bytes);
  * of both the batch holds now]. */
 int whale_batch_ell_bytes(const whale_batch* b, int64_t* out3);
 
+/* Deterministic gradients (on != 0): repeated whale_logpdf_grad calls on the
+ * same parameters give bit-identical results, as an HMC sampler's reversible
+ * trajectories need.  The reverse sweep then runs one family per workgroup
+ * with its accumulators in that workgroup's LDS, each family's partial
+ * derivatives go to its own slot (plain stores), and the slots are summed in
+ * family order; by default the partial derivatives are added atomically in
+ * whatever order the workgroups finish, which varies in the last bits
+ * (about 1e-9 relative in FP64, 1e-7 in FP32).  The reference's ForwardDiff
+ * gradient is deterministic; no counterpart. */
+int whale_batch_set_deterministic(whale_batch* b, int32_t on);
+
 /* logpdf of every family (unconditioned, src/core.jl:31-43) into
  * out_family[n_families] (may be NULL) and the conditioned batch value
  * ℓhood(Σ ℓ_i − N·condition) (src/core.jl:54,63) into *out_total. */

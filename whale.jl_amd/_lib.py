@@ -70,6 +70,7 @@ def lib():
         L.whale_batch_set_tuning.argtypes = [vp, i64p]
         L.whale_batch_set_ell_budget.argtypes = [vp, C.c_int64]
         L.whale_batch_ell_bytes.argtypes = [vp, i64p]
+        L.whale_batch_set_deterministic.argtypes = [vp, C.c_int32]
         L.whale_batch_get_tuning.argtypes = [vp, i64p]
         L.whale_batch_launches.argtypes = [vp, C.c_int32, i64p]
         L.whale_logpdf.argtypes = [vp, C.c_int32, C.c_int32, dblp, dblp]

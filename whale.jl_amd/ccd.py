@@ -203,6 +203,10 @@ class DeviceBatch:
         each family's forward and reverse sweep in one pass (whale_batch_set_ell_budget)."""
         check(_lib.lib().whale_batch_set_ell_budget(self.h, int(nbytes)), "whale_batch_set_ell_budget")
 
+    def set_deterministic(self, on: bool = True):
+        """Bit-identical gradients across calls (whale_batch_set_deterministic)."""
+        check(_lib.lib().whale_batch_set_deterministic(self.h, int(bool(on))), "whale_batch_set_deterministic")
+
     def ell_bytes(self) -> dict:
         o = np.zeros(3, np.int64)
         check(_lib.lib().whale_batch_ell_bytes(self.h, ptr(o, i64p)), "whale_batch_ell_bytes")

@@ -101,6 +101,7 @@ int plan_batch_launches(whale_batch* b);
 int ensure_gws(whale_batch* b, bool* moved);
 void drop_graphs(whale_batch* b);
 uint64_t full_bytes(const whale_batch* b);
+int replan(whale_batch* b);
 
 // LDS one wave needs for family d: its largest non-root stage (items and
 // records bounded separately) and its root stage, root records staged or not
@@ -207,6 +208,12 @@ struct whale_batch {
   size_t fused_bytes[3] = {};
   DBuf fslots;
   bool full_ok = false;  // ell, coef and ibar allocated for every family
+  // Deterministic gradients: reverse sweeps run one family per workgroup with
+  // the tables and accumulators in LDS, each family's accumulators land in its
+  // own slot of acc_fam, and a fixed-order sum over families follows, so that
+  // repeated evaluations give bit-identical gradients.
+  bool det = false;
+  DBuf acc_fam;
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   int last_mode = -1;
@@ -772,6 +779,20 @@ int whale_batch_set_ell_budget(whale_batch* b, int64_t bytes) {
   return WHALE_OK;
 }
 
+int whale_batch_set_deterministic(whale_batch* b, int32_t on) {
+  if (!b) return fail(WHALE_EINVAL, "NULL argument");
+  HIPCHK(hipSetDevice(b->device));
+  const bool was = b->det;
+  b->det = on != 0;
+  const int rc = replan(b);
+  if (rc != WHALE_OK) {
+    b->det = was;
+    (void)replan(b);
+    return rc;
+  }
+  return WHALE_OK;
+}
+
 int whale_batch_ell_bytes(const whale_batch* b, int64_t* out3) {
   if (!b || !out3) return fail(WHALE_EINVAL, "NULL argument");
   out3[0] = (int64_t)full_bytes(b);
@@ -861,11 +882,13 @@ bool plan_families(const whale_batch* b, bool fwd, int mode, const uint32_t* fam
     without = std::max(without, fam_lds(d, mode, fwd, false));
   }
   LaunchCfg a, c;
+  const bool det = b->det && !fwd;  // deterministic reverse sweep: one family per workgroup, tables in LDS
+  if (det) multi = false;
   const bool oa = !force_gws && plan_launch(b->model, fwd, mode, with, grid_fams, multi, &a, b->tune.waves_cap);
   const bool oc = !force_gws && plan_launch(b->model, fwd, mode, without, grid_fams, multi, &c, b->tune.waves_cap);
   if (oa && (!oc || a.waves_per_cu >= c.waves_per_cu)) *out = a;
   else if (oc) *out = c;
-  if (oa || oc) return true;
+  if ((oa || oc) && (!det || out->tabl)) return true;
   // the stage buffers in global memory: one wave per workgroup, root records
   // read from global memory, only the node constants (and accumulators) in LDS
   const whale_model* m = b->model;
@@ -873,11 +896,11 @@ bool plan_families(const whale_batch* b, bool fwd, int mode, const uint32_t* fam
   LaunchCfg g;
   g.gws = true;
   g.fpw = 1;
-  g.tabl = false;
+  g.tabl = det;
   g.waves_per_cu = 1;
   g.L.wave_bytes = (uint32_t)align16(without);
   g.L.shared_bytes = (uint32_t)(fwd ? fwd_shared_bytes(mode, false, m->h.tab_rows, m->h.n, m->h.n_wgd)
-                                    : bwd_shared_bytes(mode, false, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd));
+                                    : bwd_shared_bytes(mode, det, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd));
   if (g.L.shared_bytes > m->ctx->lds_wg || without > 0xffffffffull) return false;
   g.grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)m->ctx->n_cu, grid_fams));
   *out = g;
@@ -1039,16 +1062,17 @@ bool plan_fused_launch(const whale_batch* b, int mode, const uint32_t* fams, siz
   }
   const whale_model* m = b->model;
   LaunchCfg a, c;
+  if (b->det) multi = false;
   const bool oa = !force_gws && plan_launch(m, false, mode, with, grid_fams, multi, &a, b->tune.waves_cap);
   const bool oc = !force_gws && plan_launch(m, false, mode, without, grid_fams, multi, &c, b->tune.waves_cap);
   if (oa && (!oc || a.waves_per_cu >= c.waves_per_cu)) *out = a;
   else if (oc) *out = c;
-  if (oa || oc) return true;
+  if ((oa || oc) && (!b->det || out->tabl)) return true;
   const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   LaunchCfg g;
   g.gws = true;
   g.fpw = 1;
-  g.tabl = false;
+  g.tabl = false;  // the fused kernel's global-memory form keeps the tables in global memory
   g.waves_per_cu = 1;
   g.L.wave_bytes = (uint32_t)align16(without);
   g.L.shared_bytes = (uint32_t)bwd_shared_bytes(mode, false, m->h.tab_rows, AL.n, m->h.n, m->h.n_wgd);
@@ -1161,7 +1185,8 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   hipStream_t st = c->stream;
   const BatchDev B = b->dev();
   const ModelDev M = m->dev();
-  const WorkRaw W = ws ? *ws : b->work();
+  WorkRaw W = ws ? *ws : b->work();
+  W.acc_fam = b->det && grad ? b->acc_fam.as<double>() : nullptr;
   const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
   HIPCHK(launch_fill_u32(st, W.ticket, 0u, kTickets));
@@ -1377,6 +1402,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   if (routed) HIPCHK(hipStreamWaitEvent(st, c->fork[7], 0));
   if (timed) HIPCHK(hipEventRecord(c->ev[3], st));
   }
+  if (W.acc_fam) HIPCHK(launch_det_sum(st, W.acc_fam, nf, AL.n, W.acc));
   double* out = ws_out ? ws_out : m->out.as<double>();
   HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
                      m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out));
@@ -1412,6 +1438,10 @@ int run(whale_batch* b, int32_t mode_in, int32_t launch, bool grad, double* out_
     drop_graphs(b);
   }
   const bool fused = use_fused(b, launch, keep);
+  if (b->det && grad) {
+    const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
+    HIPCHK(b->acc_fam.alloc(sizeof(double) * (size_t)b->n_fam * AL.n));
+  }
   if (fused) {
     if (!b->fused_ok[mode]) {
       const int rc = plan_fused(b, mode);

@@ -53,6 +53,7 @@ struct WorkRaw {
   uint32_t* flag;     // per family: 1 if the FP32 form left its range (recomputed in FP64)
   uint32_t* fb;       // fb[0] = number of such families, fb[1..] = their indices
   unsigned char* gws; // stage buffers of launches whose families do not fit LDS (LaunchCfg::gws)
+  double* acc_fam = nullptr;  // deterministic gradients: per-family accumulator slots
 };
 
 // gradient accumulator layout (doubles)
@@ -153,6 +154,7 @@ hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, 
                        double* ebar, const double* drow, const double* dnode, const int32_t* dlevels,
                        const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int n);
+hipError_t launch_det_sum(hipStream_t st, const double* slots, uint32_t n_fam, int ncol, double* out);
 hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n);
 
 // What the backtracking sampler reads: the stored ℓ (FP32 or FP64 values,

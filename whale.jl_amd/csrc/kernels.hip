@@ -67,9 +67,10 @@ struct WorkDev {
   uint32_t* flag;
   uint32_t* fb;
   unsigned char* gws;
+  double* acc_fam;    // deterministic gradients: per-family accumulator slots (null otherwise)
   explicit WorkDev(const WorkRaw& w)
       : ell((Real*)w.ell), coef((Real*)w.coef), E(w.E), ibar((Real*)w.ibar), fam_ll(w.fam_ll), acc(w.acc),
-        ticket(w.ticket), flag(w.flag), fb(w.fb), gws(w.gws) {}
+        ticket(w.ticket), flag(w.flag), fb(w.fb), gws(w.gws), acc_fam(w.acc_fam) {}
 };
 
 __device__ __forceinline__ double eps_top(const ModelDev& M, int node) {
@@ -1293,6 +1294,15 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
                                            uint32_t coef_base) {
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
+  // Deterministic gradients: this family's accumulators go to its own slot
+  // with plain stores (one wave per workgroup, tables and accumulators in LDS,
+  // each table row flushed once per family); a fixed-order sum over the
+  // families follows.  Otherwise they go to the batch sums with atomics.
+  double* const accF = TabL && W.acc_fam ? W.acc_fam + (size_t)f * AL.n : nullptr;
+  if (accF) {
+    for (int j = (int)lane; j < AL.n; j += kWG) accF[j] = 0.0;
+    wave_sync_mem();
+  }
   const FamDev F = B.fam[f];
   auto coef_at = [&](uint32_t k) -> Real { return Scaled ? coefs[k - coef_base] : (Real)B.rec_p[k]; };
   // ---------------- root
@@ -1659,8 +1669,13 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
           const Real b = __hip_atomic_exchange(&accPS[AL.psi + tab + r], Real(0), __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
-          if (a != Real(0)) unsafeAtomicAdd(W.acc + AL.phi + tab + r, (double)a);
-          if (b != Real(0)) unsafeAtomicAdd(W.acc + AL.psi + tab + r, (double)b);
+          if (accF) {
+            accF[AL.phi + tab + r] = (double)a;
+            accF[AL.psi + tab + r] = (double)b;
+          } else {
+            if (a != Real(0)) unsafeAtomicAdd(W.acc + AL.phi + tab + r, (double)a);
+            if (b != Real(0)) unsafeAtomicAdd(W.acc + AL.psi + tab + r, (double)b);
+          }
         }
       }
     }
@@ -1758,6 +1773,15 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
       if (tp.kind == WHALE_NODE_SPECIATION) ibar[tp.ib + tp.nc] = bB[i];
     });
   }
+  if (accF) {
+    // ε, q and the root constants of this family, from the workgroup's LDS
+    wsync<Gws>();
+    for (int j = AL.eps + (int)lane; j < AL.n; j += kWG) {
+      accF[j] = accX[j];
+      accX[j] = 0.0;
+    }
+    wsync<Gws>();
+  }
 }
 
 template <typename Real, bool Scaled, bool TabL, bool Gws>
@@ -1812,7 +1836,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
     bwd_family<Real, Scaled, TabL, Gws>(B, M, W, L, f, wb, sTab, accPS, accX, NC, lane, W.ell + B.fam[f].ell0,
                                         W.ibar + B.fam[f].ibar0, coefs, 0u);
   }
-  if (TabL) {
+  if (TabL && !W.acc_fam) {
     __syncthreads();
     for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) {
       const Real v = accPS[j];
@@ -1882,7 +1906,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_fused(BatchDev B, ModelDev M, 
     wave_sync_mem();  // the forward's rows and coefficients are visible to the wave
     bwd_family<Real, Scaled, TabL, Gws>(B, M, W, L, f, wb, sTab, accPS, accX, NC, lane, ell, ibar, coef, rec0);
   }
-  if (TabL && grad) {
+  if (TabL && grad && !W.acc_fam) {
     __syncthreads();
     for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) {
       const Real v = accPS[j];
@@ -2059,7 +2083,10 @@ static hipError_t bwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev
   const WorkDev<R> w(W);
   if (C.fpw < 1 || C.fpw > kMaxFPW) return hipErrorInvalidValue;
   if (C.gws) {
-    if (C.fpw != 1 || C.tabl || !W.gws) return hipErrorInvalidValue;
+    if (C.fpw != 1 || !W.gws) return hipErrorInvalidValue;
+    if (C.tabl)
+      return go(k_backward<R, S, true, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+                fam_count);
     return go(k_backward<R, S, false, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
               fam_count);
   }
@@ -2377,6 +2404,24 @@ hipError_t launch_backtrack(hipStream_t st, const BtDev& D, const ModelDev& M, u
   const uint32_t grid = (uint32_t)((lanes + 63) / 64);
   hipLaunchKernelGGL(k_backtrack, dim3(grid), dim3(64), 0, st, D, M, fam_begin, n_fams, n_samples, seed, cap, counts,
                      nodes, stack);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k_det_sum: the batch's gradient accumulators as the sum of the per-family
+// slots in family order (one lane per accumulator), so that the result does
+// not depend on which workgroup evaluated which family, or when.
+// ---------------------------------------------------------------------------
+__global__ void k_det_sum(const double* slots, uint32_t n_fam, int ncol, double* out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ncol) return;
+  double s = 0.0;
+  for (uint32_t f = 0; f < n_fam; ++f) s += slots[(size_t)f * ncol + j];
+  out[j] = s;
+}
+
+hipError_t launch_det_sum(hipStream_t st, const double* slots, uint32_t n_fam, int ncol, double* out) {
+  hipLaunchKernelGGL(k_det_sum, dim3((ncol + 255) / 256), dim3(256), 0, st, slots, n_fam, ncol, out);
   return hipGetLastError();
 }
 
