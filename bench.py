@@ -40,7 +40,8 @@ def measured_traffic(kernel):
     workload, from the FETCH_SIZE/WRITE_SIZE profile committed under profiles/
     (copy-calibrated), or None when the profile is absent."""
     path = os.path.join(ROOT, "profiles", "config4_fp32_gradient_hbm_traffic.md")
-    sym = {"k_backward": "`k_backward<float,true,true>`", "k_forward": "`k_forward<float,true,true,true>`"}[kernel]
+    sym = {"k_backward": "`k_backward<float,true,true,false>`",
+           "k_forward": "`k_forward<float,true,true,true,false>`"}[kernel]
     try:
         for line in open(path, encoding="utf-8"):
             if line.startswith("| " + sym + " |"):
@@ -105,12 +106,26 @@ def sampler(w, synth, workload="config5", n_points=256, n_families=None, tune_ca
         lat.append(time.perf_counter() - t)
     total = time.perf_counter() - t0
     lat = np.array(lat) * 1e3
+    # the same loop with bit-identical gradients (reversible trajectories)
+    batch.set_deterministic(True)
+    for m in models[:3]:
+        w.gradient(m, ccds, mode="fp32")
+    lat_det = []
+    for m in models[:64]:
+        t = time.perf_counter()
+        w.gradient(m, ccds, mode="fp32")
+        lat_det.append(time.perf_counter() - t)
+    batch.set_deterministic(False)
+    lat_det = np.array(lat_det) * 1e3
     out = {"workload": f"{wl.name}: {n_fam} families, {len(points)} parameter points back to back, "
                        "logL+gradient FP32-scaled, device-resident CCDs",
            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
            "evals_per_s": len(points) / total, "families_per_s": n_fam * len(points) / total,
+           "deterministic_p50_ms": float(np.percentile(lat_det, 50)),
+           "deterministic_p99_ms": float(np.percentile(lat_det, 99)),
            "tuning": tuned}
-    log(f"sampler: p50 {out['p50_ms']:.3f} ms, p99 {out['p99_ms']:.3f} ms, {out['evals_per_s']:.1f} evals/s")
+    log(f"sampler: p50 {out['p50_ms']:.3f} ms, p99 {out['p99_ms']:.3f} ms, {out['evals_per_s']:.1f} evals/s; "
+        f"deterministic p50 {out['deterministic_p50_ms']:.3f} ms")
     return out
 
 

@@ -10,7 +10,9 @@
         counters per kernel from a --pmc run: FETCH_SIZE / WRITE_SIZE scaled
         by a 256 MiB device-to-device copy in the same run (bench.py
         --copy-probe) to bytes per evaluation, and SQ_* counters as sums per
-        evaluation with the derived busy / wait / VALU / LDS ratios.
+        evaluation with the derived busy / wait / VALU / LDS ratios;
+    python whale.jl_amd/rocprof_summary.py traffic FETCH_DIR WRITE_DIR
+        the two byte counters' runs combined into MB per evaluation.
 
 DIR is the -d directory of the rocprofv3 run, searched recursively for its
 result databases (rocpd SQLite, the default output format of rocprofv3).
@@ -202,7 +204,59 @@ def pmc(d):
             print(f"| `{k}` | " + " | ".join(f"{c[a] / c[b]:.3f}" if c[b] else "–" for _, a, b in cols) + " |")
 
 
+def _fp32_sums(d):
+    """Per kernel: counter sums over the FP32 logL+gradient evaluations of a
+    --pmc run, the number of such evaluations, and the copy's sums."""
+    disp = {}
+    for name, di, cname, val, start in _query(d, "select kernel_name, dispatch_id, counter_name, value, start "
+                                                 "from counters_collection"):
+        x = disp.setdefault(di, dict(name=short_name(name), start=int(start), c=defaultdict(float)))
+        x["c"][cname] += float(val)
+    order = sorted(disp.values(), key=lambda x: x["start"])
+    evs, cur = [], None
+    for x in order:
+        if x["name"].startswith("k_tables"):
+            cur = []
+            evs.append(cur)
+        if cur is not None:
+            cur.append(x)
+    fp32 = [ev for ev in evs if any(x["name"].startswith("k_backward<float") for x in ev)]
+    per, calls, copy = defaultdict(lambda: defaultdict(float)), defaultdict(int), defaultdict(float)
+    for ev in fp32:
+        for x in ev:
+            calls[x["name"]] += 1
+            for c, v in x["c"].items():
+                per[x["name"]][c] += v
+    for x in order:
+        if x["name"].startswith("__amd_rocclr_copyBuffer"):
+            for c, v in x["c"].items():
+                copy[c] += v
+    return per, calls, max(1, len(fp32)), copy
+
+
+def traffic(fetch_dir, write_dir):
+    """FETCH_SIZE and WRITE_SIZE runs combined: MB per FP32 evaluation and kernel."""
+    pf, cf, nf, copy_f = _fp32_sums(fetch_dir)
+    pw, cw, nw, copy_w = _fp32_sums(write_dir)
+    sf = 256 * 1024 * 1024 / (copy_f["FETCH_SIZE"] * 1024)
+    sw = 256 * 1024 * 1024 / (copy_w["WRITE_SIZE"] * 1024)
+    print(f"Calibration: the 256 MiB device-to-device copy reads FETCH_SIZE {copy_f['FETCH_SIZE']:.1f} KiB "
+          f"and WRITE_SIZE {copy_w['WRITE_SIZE']:.1f} KiB: scales {sf:.3f} and {sw:.3f} (bytes per counted byte).")
+    print(f"FP32 logL+gradient evaluations: {nf} in the FETCH_SIZE run, {nw} in the WRITE_SIZE run.\n")
+    print("| kernel | launches per evaluation | FETCH (MB) | WRITE (MB) | total (MB) |")
+    print("|---|---|---|---|---|")
+    for k in sorted(set(pf) | set(pw)):
+        if k.startswith("__amd"):
+            continue
+        f = pf[k]["FETCH_SIZE"] * 1024 * sf / nf / 1e6
+        w_ = pw[k]["WRITE_SIZE"] * 1024 * sw / nw / 1e6
+        print(f"| `{k}` | {cf[k] / nf:g} | {f:.1f} | {w_:.1f} | {f + w_:.1f} |")
+
+
 if __name__ == "__main__":
-    if len(sys.argv) != 3 or sys.argv[1] not in ("trace", "pmc"):
+    if len(sys.argv) == 4 and sys.argv[1] == "traffic":
+        traffic(sys.argv[2], sys.argv[3])
+    elif len(sys.argv) == 3 and sys.argv[1] in ("trace", "pmc"):
+        {"trace": trace, "pmc": pmc}[sys.argv[1]](sys.argv[2])
+    else:
         raise SystemExit(__doc__)
-    {"trace": trace, "pmc": pmc}[sys.argv[1]](sys.argv[2])
