@@ -62,6 +62,19 @@ def _launches(d):
     return out
 
 
+def _evaluations(launches):
+    """Launches in start order split into evaluations: each ends with its
+    k_tail (the batch sum), and holds what was launched since the previous
+    one (its k_tables when parameters were set, its fills and DP kernels)."""
+    evs, cur = [], []
+    for x in launches:
+        cur.append(x)
+        if x["name"].startswith("k_tail"):
+            evs.append(cur)
+            cur = []
+    return evs
+
+
 def _union(iv):
     iv = sorted(iv)
     tot, cs, ce = 0, None, None
@@ -92,14 +105,7 @@ def trace(d):
     for k, v in sorted(g.items(), key=lambda kv: -sum(kv[1]) / len(kv[1])):
         print(f"| `{k[0]}` | {k[1]} | {k[2]} | {k[3]} | {len(v)} | {sum(v) / len(v):.3f} | {min(v):.3f} | "
               f"{max(v):.3f} |")
-    # evaluations: from one k_tables launch to the next
-    evs, cur = [], None
-    for x in L:
-        if x["name"].startswith("k_tables"):
-            cur = []
-            evs.append(cur)
-        if cur is not None:
-            cur.append(x)
+    evs = _evaluations(L)
     res = {"apart": defaultdict(list), "overlapped": defaultdict(list)}
     for ev in evs:
         fwd = [x for x in ev if x["name"].startswith("k_forward")]
@@ -147,13 +153,7 @@ def pmc(d):
         x = disp.setdefault(di, dict(name=short_name(name), start=int(start), c=defaultdict(float)))
         x["c"][cname] += float(val)
     order = sorted(disp.values(), key=lambda x: x["start"])
-    evs, cur = [], None
-    for x in order:
-        if x["name"].startswith("k_tables"):
-            cur = []
-            evs.append(cur)
-        if cur is not None:
-            cur.append(x)
+    evs = _evaluations(order)
     fp32 = [ev for ev in evs if any(x["name"].startswith("k_backward<float") for x in ev)]
     per = defaultdict(lambda: defaultdict(float))  # kernel -> counter -> sum over FP32 evaluations
     calls = defaultdict(int)
@@ -213,13 +213,7 @@ def _fp32_sums(d):
         x = disp.setdefault(di, dict(name=short_name(name), start=int(start), c=defaultdict(float)))
         x["c"][cname] += float(val)
     order = sorted(disp.values(), key=lambda x: x["start"])
-    evs, cur = [], None
-    for x in order:
-        if x["name"].startswith("k_tables"):
-            cur = []
-            evs.append(cur)
-        if cur is not None:
-            cur.append(x)
+    evs = _evaluations(order)
     fp32 = [ev for ev in evs if any(x["name"].startswith("k_backward<float") for x in ev)]
     per, calls, copy = defaultdict(lambda: defaultdict(float)), defaultdict(int), defaultdict(float)
     for ev in fp32:
