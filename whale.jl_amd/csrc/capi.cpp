@@ -489,7 +489,7 @@ int whale_model_create(whale_ctx* ctx, int32_t n, const int32_t* parent, const i
   HIPCHK(m->drow.alloc(sizeof(double) * 9 * h.tab_rows));
   HIPCHK(m->dnode.alloc(sizeof(double) * 4 * n));
   HIPCHK(m->par.alloc(sizeof(double) * (3 * n + nwgd + 1)));
-  HIPCHK(m->ebar.alloc(sizeof(double) * n));
+  HIPCHK(m->ebar.alloc(sizeof(double) * 7 * n));  // ε̄ per node, then the tail's 6 coefficients per node
   HIPCHK(m->out.alloc(sizeof(double) * (2 + 2 * n + nwgd)));
   *out = m.release();
   return WHALE_OK;

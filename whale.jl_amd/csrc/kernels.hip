@@ -118,85 +118,163 @@ __device__ __forceinline__ void xprod(Real p, Real v1, int e1, Real v2, int e2, 
 }
 
 // ---------------------------------------------------------------------------
-// k_tables: one workgroup; nodes by height (children first), one lane per node
+// k_tables: one workgroup.  The slice map over Δt is the Möbius map
+//   ε ← (a + (1 − a − b) ε) / (1 − b ε)
+// (the linear birth-death extinction pgf, src/model.jl:162-191); r slices
+// compose to a map of the same kind, which eps_after evaluates in closed
+// form.  The tops of the branches go level by level (a node's bottom value is
+// a function of its children's tops), one lane per node; then every row of
+// every branch is independent: ε(r−1) in closed form, then the last step and
+// the row's φ, ψ and derivatives as the slice recursion writes them.
 // ---------------------------------------------------------------------------
 // deriv layout per table row (9 doubles): dε/dα, dε/dβ, dε/dε', dφ/dα, dφ/dβ,
 // dφ/dε', dψ/dα, dψ/dβ, dψ/dε'.  Per node (4 doubles): dα/dλ, dα/dμ, dβ/dλ, dβ/dμ.
-__global__ void k_tables(ModelDev M, double* eps, double* phi, double* psi, double* drow, double* dnode,
-                         const int32_t* hlevels, const int32_t* hnodes, int n_levels) {
-  const double* lam = M.par;
-  const double* mu = M.par + M.n;
+struct SliceAB {
+  double a, b, dal, dam, dbl, dbm;
+};
+
+// α, β of one slice of length t (and their derivatives in λ, μ)
+__device__ inline SliceAB slice_ab(double l, double u, double t) {
+  SliceAB s{0, 0, 0, 0, 0, 0};
+  bool close = (l == u) || (isfinite(l) && isfinite(u) && fabs(l - u) <= 1e-6);
+  if (close) {
+    double x = 1.0 + l * t;
+    s.a = l * t / x;
+    s.dal = t / (x * x);
+    s.dam = 0.0;
+  } else {
+    double E = exp(t * (l - u));
+    double N = u * (E - 1.0), D = l * E - u;
+    s.a = N / D;
+    double dNl = u * t * E, dNm = (E - 1.0) - u * t * E;
+    double dDl = E + l * t * E, dDm = -l * t * E - 1.0;
+    s.dal = (dNl * D - N * dDl) / (D * D);
+    s.dam = (dNm * D - N * dDm) / (D * D);
+  }
+  s.b = (l / u) * s.a;
+  s.dbl = s.a / u + (l / u) * s.dal;
+  s.dbm = -l * s.a / (u * u) + (l / u) * s.dam;
+  return s;
+}
+
+// ε after k slices from e0.  The map has the fixed points 1 and ρ = a/b
+// (= μ/λ), and z = 1/(1 − ε) moves affinely: z ← κz + b/(1 − a) with
+// κ = (1 − b)/(1 − a).  With δ = κ − 1, S = (κ^k − 1)/δ:
+//   ε_k = (e0 + S(a − b·e0)/(1 − a)) / (κ^k + S·b(1 − e0)/(1 − a)),
+// both sums of non-negative terms while e0 ≤ ρ; for e0 > ρ (a leaf whose
+// sampling probability p exceeds μ/λ)
+//   (ε_k − ρ)/(1 − ε_k) = κ^k (e0 − ρ)/(1 − e0)
+// keeps ε_k = ρ + (excess) free of cancellation.  κ^k and S go through
+// log1p/expm1 (κ is near 1 for short slices) and are scaled by κ^−k when
+// κ > 1.  Accurate to a few ulp, where k dependent steps drift by k ulp.
+__device__ inline double eps_after(double a, double b, int k, double e0) {
+  if (k == 0 || e0 == 1.0) return e0;
+  const double om = 1.0 - a;
+  if (om == 0.0) return 1.0;
+  const double d = (a - b) / om;
+  if (a < b * e0) {
+    const double rho = a / b;
+    const double W = exp((double)k * log1p(d)) * (e0 - rho) / (1.0 - e0);
+    return (rho + W) / (1.0 + W);
+  }
+  const double al = (a - b * e0) / om, be = b * (1.0 - e0) / om;
+  if (d == 0.0) return (e0 + (double)k * al) / (1.0 + (double)k * be);
+  const double L = (double)k * log1p(d);
+  if (L <= 0.0) {
+    const double S = expm1(L) / d;
+    return (e0 + S * al) / (exp(L) + S * be);
+  }
+  const double SK = -expm1(-L) / d;  // S/κ^k
+  return (e0 * exp(-L) + SK * al) / (1.0 + SK * be);
+}
+
+// one slice step from ε' = ep (the row's ε, φ, ψ)
+__device__ inline void slice_step(double a, double b, double ep, double* en, double* ph, double* ps, double* den_) {
+  const double den = 1.0 - b * ep;
+  const double num = a + (1.0 - a - b) * ep;
+  *en = num / den;
+  *ph = (1.0 - a) * (1.0 - b) / (den * den);
+  *ps = (1.0 - a) * (1.0 - b) * b / (den * den * den);
+  *den_ = den;
+}
+
+__device__ inline double node_e0(const ModelDev& M, const NodeDev& d, int e) {
   const double* p = M.par + 2 * M.n;
   const double* q = M.par + 3 * M.n;
+  if (d.kind == WHALE_NODE_WGD) {
+    double ec = eps_top(M, d.child0);
+    double qq = q[d.wgd];
+    return qq * (ec * ec) + (1.0 - qq) * ec;
+  }
+  if (d.kind == WHALE_NODE_LEAF) return p[e];
+  return eps_top(M, d.child0) * eps_top(M, d.child1);
+}
+
+__global__ __launch_bounds__(1024) void k_tables(ModelDev M, double* eps, double* phi, double* psi, double* drow, double* dnode,
+                         const int32_t* hlevels, const int32_t* hnodes, int n_levels) {
+  extern __shared__ int32_t s_tab[];  // first table row of each node, then tab_rows
+  const double* lam = M.par;
+  const double* mu = M.par + M.n;
+  for (int e = threadIdx.x; e < M.n; e += blockDim.x) s_tab[e] = M.node[e].tab;
+  if (threadIdx.x == 0) s_tab[M.n] = M.tab_rows;
+  // tops, children first
   for (int L = 0; L < n_levels; ++L) {
     for (int k = hlevels[L] + threadIdx.x; k < hlevels[L + 1]; k += blockDim.x) {
-      int e = hnodes[k];
+      const int e = hnodes[k];
       const NodeDev& d = M.node[e];
-      int t0 = d.tab;
-      double e0;
-      if (d.kind == WHALE_NODE_WGD) {
-        double ec = eps_top(M, d.child0);
-        double qq = q[d.wgd];
-        e0 = qq * (ec * ec) + (1.0 - qq) * ec;
-      } else if (d.kind == WHALE_NODE_LEAF) {
-        e0 = p[e];
-      } else {
-        e0 = eps_top(M, d.child0) * eps_top(M, d.child1);
-      }
+      const int t0 = d.tab;
+      const double e0 = node_e0(M, d, e);
       eps[t0] = e0;
       phi[t0] = 1.0;
       psi[t0] = 1.0;
-      double l = lam[e], u = mu[e], t = d.delta;
-      double a = 0, b = 0, dal = 0, dam = 0, dbl = 0, dbm = 0;
+      SliceAB s{0, 0, 0, 0, 0, 0};
+      if (d.R > 1) s = slice_ab(lam[e], mu[e], d.delta);
+      dnode[4 * e + 0] = s.dal;
+      dnode[4 * e + 1] = s.dam;
+      dnode[4 * e + 2] = s.dbl;
+      dnode[4 * e + 3] = s.dbm;
       if (d.R > 1) {
-        bool close = (l == u) || (isfinite(l) && isfinite(u) && fabs(l - u) <= 1e-6);
-        if (close) {
-          double x = 1.0 + l * t;
-          a = l * t / x;
-          dal = t / (x * x);
-          dam = 0.0;
-        } else {
-          double E = exp(t * (l - u));
-          double N = u * (E - 1.0), D = l * E - u;
-          a = N / D;
-          double dNl = u * t * E, dNm = (E - 1.0) - u * t * E;
-          double dDl = E + l * t * E, dDm = -l * t * E - 1.0;
-          dal = (dNl * D - N * dDl) / (D * D);
-          dam = (dNm * D - N * dDm) / (D * D);
-        }
-        b = (l / u) * a;
-        dbl = a / u + (l / u) * dal;
-        dbm = -l * a / (u * u) + (l / u) * dam;
-      }
-      dnode[4 * e + 0] = dal;
-      dnode[4 * e + 1] = dam;
-      dnode[4 * e + 2] = dbl;
-      dnode[4 * e + 3] = dbm;
-      double ep = e0;
-      for (int r = 1; r < d.R; ++r) {
-        double den = 1.0 - b * ep;
-        double num = a + (1.0 - a - b) * ep;
-        double en = num / den;
-        double ph = (1.0 - a) * (1.0 - b) / (den * den);
-        double ps = (1.0 - a) * (1.0 - b) * b / (den * den * den);
-        eps[t0 + r] = en;
-        phi[t0 + r] = ph;
-        psi[t0 + r] = ps;
-        double* g = drow + 9 * (size_t)(t0 + r);
-        double d2 = den * den, d3 = d2 * den, d4 = d3 * den;
-        g[0] = (1.0 - ep) / den;
-        g[1] = ep * (en - 1.0) / den;
-        g[2] = ((1.0 - a - b) + b * en) / den;
-        g[3] = -(1.0 - b) / d2;
-        g[4] = -(1.0 - a) / d2 + (1.0 - a) * (1.0 - b) * 2.0 * ep / d3;
-        g[5] = (1.0 - a) * (1.0 - b) * 2.0 * b / d3;
-        g[6] = -(1.0 - b) * b / d3;
-        g[7] = (1.0 - a) * (1.0 - 2.0 * b) / d3 + 3.0 * ep * (1.0 - a) * (1.0 - b) * b / d4;
-        g[8] = (1.0 - a) * (1.0 - b) * b * 3.0 * b / d4;
-        ep = en;
+        double en, ph, ps, den;
+        slice_step(s.a, s.b, eps_after(s.a, s.b, d.R - 2, e0), &en, &ph, &ps, &den);
+        eps[t0 + d.R - 1] = en;
       }
     }
     __syncthreads();
+  }
+  // every row r ≥ 1 of every branch
+  for (int t = threadIdx.x; t < M.tab_rows; t += blockDim.x) {
+    int lo = 0, hi = M.n;  // the node e with s_tab[e] ≤ t < s_tab[e + 1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s_tab[mid] <= t) lo = mid;
+      else hi = mid;
+    }
+    const int e = lo;
+    const int r = t - s_tab[e];
+    if (r == 0) continue;
+    const NodeDev& d = M.node[e];
+    const SliceAB s = slice_ab(lam[e], mu[e], d.delta);
+    const double a = s.a, b = s.b;
+    const double ep = eps_after(a, b, r - 1, eps[d.tab]);
+    double en, ph, ps, den;
+    slice_step(a, b, ep, &en, &ph, &ps, &den);
+    if (r == d.R - 1)
+      en = eps[t];  // the top, as the levels wrote it for the parent
+    else
+      eps[t] = en;
+    phi[t] = ph;
+    psi[t] = ps;
+    double* g = drow + 9 * (size_t)t;
+    double d2 = den * den, d3 = d2 * den, d4 = d3 * den;
+    g[0] = (1.0 - ep) / den;
+    g[1] = ep * (en - 1.0) / den;
+    g[2] = ((1.0 - a - b) + b * en) / den;
+    g[3] = -(1.0 - b) / d2;
+    g[4] = -(1.0 - a) / d2 + (1.0 - a) * (1.0 - b) * 2.0 * ep / d3;
+    g[5] = (1.0 - a) * (1.0 - b) * 2.0 * b / d3;
+    g[6] = -(1.0 - b) * b / d3;
+    g[7] = (1.0 - a) * (1.0 - 2.0 * b) / d3 + 3.0 * ep * (1.0 - a) * (1.0 - b) * b / d4;
+    g[8] = (1.0 - a) * (1.0 - b) * b * 3.0 * b / d4;
   }
 }
 
@@ -1920,13 +1998,49 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_fused(BatchDev B, ModelDev M, 
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_tail(ModelDev M, const double* fam_ll, uint32_t n_fam, const double* acc_in, double* ebar,
-                       const double* drow, const double* dnode, const int32_t* dlevels, const int32_t* dnodes,
-                       int n_dlevels, int cond, int with_grad, double* out) {
-  __shared__ double red[256];
+constexpr int kTailThreads = 1024;
+
+// k_tail: one workgroup.  Σℓ over the families, the conditioning term, and
+// the chain rule from the per-row table adjoints to ∂λ, ∂μ, ∂q, ∂η.  Down a
+// branch the table adjoint is affine in the adjoint of its top ε:
+//   ε̄(r−1) = ε̄(r)·∂ε/∂ε' + φ̄_r·∂φ/∂ε' + ψ̄_r·∂ψ/∂ε',
+// and so are ᾱ and β̄.  Each node's three affine maps are built from its rows
+// with one lane per node, all nodes at once; only applying them runs by depth
+// (parents first, a node's top adjoint takes its parent's bottom one).
+// Per node in cf (6 doubles): ε̄(bottom), ᾱ, β̄ as (slope, offset) pairs in ε̄(top).
+__global__ __launch_bounds__(kTailThreads) void k_tail(ModelDev M, const double* fam_ll, uint32_t n_fam,
+                                                       const double* acc_in, double* ebar, const double* drow,
+                                                       const double* dnode, const int32_t* dlevels,
+                                                       const int32_t* dnodes, int n_dlevels, int cond, int with_grad,
+                                                       double* out) {
+  __shared__ double red[kTailThreads];
   __shared__ double s_total, s_eta;
   const int t = threadIdx.x;
   const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
+  double* cf = ebar + M.n;
+  // the affine maps first: they do not depend on the sum
+  if (with_grad)
+    for (int e = t; e < M.n; e += blockDim.x) {
+      const NodeDev& d = M.node[e];
+      double ga = 1.0, gb = 0.0;           // ε̄ = ga·ε̄(top) + gb
+      double aa = 0.0, ab = 0.0, ba = 0.0, bb = 0.0;
+      for (int r = d.R - 1; r >= 1; --r) {
+        const double* G = drow + 9 * (size_t)(d.tab + r);
+        const double fp = acc_in[AL.phi + d.tab + r], fs = acc_in[AL.psi + d.tab + r];
+        aa += ga * G[0];
+        ab += gb * G[0] + fp * G[3] + fs * G[6];
+        ba += ga * G[1];
+        bb += gb * G[1] + fp * G[4] + fs * G[7];
+        gb = gb * G[2] + fp * G[5] + fs * G[8];
+        ga = ga * G[2];
+      }
+      cf[6 * e + 0] = ga;
+      cf[6 * e + 1] = gb;
+      cf[6 * e + 2] = aa;
+      cf[6 * e + 3] = ab;
+      cf[6 * e + 4] = ba;
+      cf[6 * e + 5] = bb;
+    }
   // Σ ℓ_i
   double s = 0;
   for (uint32_t i = t; i < n_fam; i += blockDim.x) s += fam_ll[i];
@@ -1992,18 +2106,13 @@ __global__ void k_tail(ModelDev M, const double* fam_ll, uint32_t n_fam, const d
     for (int k = dlevels[L] + t; k < dlevels[L + 1]; k += blockDim.x) {
       const int e = dnodes[k];
       const NodeDev& d = M.node[e];
-      double g = ebar[e];
-      double ga = 0, gb = 0;
-      for (int r = d.R - 1; r >= 1; --r) {
-        const double* G = drow + 9 * (size_t)(d.tab + r);
-        const double fp = acc_in[AL.phi + d.tab + r], fs = acc_in[AL.psi + d.tab + r];
-        ga += g * G[0] + fp * G[3] + fs * G[6];
-        gb += g * G[1] + fp * G[4] + fs * G[7];
-        g = g * G[2] + fp * G[5] + fs * G[8];
-      }
+      const double gt = ebar[e];
+      const double* c = cf + 6 * e;
+      const double g = c[0] * gt + c[1];  // ε̄ of row 0
+      const double ga = c[2] * gt + c[3], gb = c[4] * gt + c[5];
       dl[e] = d.R > 1 ? ga * dnode[4 * e + 0] + gb * dnode[4 * e + 2] : 0.0;
       dm[e] = d.R > 1 ? ga * dnode[4 * e + 1] + gb * dnode[4 * e + 3] : 0.0;
-      // row 1 of ε: product of the children's tops, or the WGD retention map
+      // row 0 of ε: product of the children's tops, or the WGD retention map
       if (d.kind == WHALE_NODE_WGD) {
         const double ec = eps_top(M, d.child0);
         const double qq = M.par[3 * M.n + d.wgd];
@@ -2148,14 +2257,15 @@ hipError_t launch_fused(int mode, const LaunchCfg& C, hipStream_t st, const Batc
 
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
                          double* dnode, const int32_t* hlevels, const int32_t* hnodes, int n_levels) {
-  hipLaunchKernelGGL(k_tables, dim3(1), dim3(256), 0, st, M, eps, phi, psi, drow, dnode, hlevels, hnodes, n_levels);
+  hipLaunchKernelGGL(k_tables, dim3(1), dim3(1024), sizeof(int32_t) * (M.n + 1), st, M, eps, phi, psi, drow, dnode,
+                     hlevels, hnodes, n_levels);
   return hipGetLastError();
 }
 
 hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,
                        double* ebar, const double* drow, const double* dnode, const int32_t* dlevels,
                        const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out) {
-  hipLaunchKernelGGL(k_tail, dim3(1), dim3(256), 0, st, M, fam_ll, n_fam, acc, ebar, drow, dnode, dlevels, dnodes,
+  hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), 0, st, M, fam_ll, n_fam, acc, ebar, drow, dnode, dlevels, dnodes,
                      n_dlevels, cond, with_grad, out);
   return hipGetLastError();
 }
