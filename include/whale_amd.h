@@ -191,6 +191,11 @@ int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t
  * >= 1024), layout 0..5, waves per CU 1..32].  No reference counterpart. */
 int whale_batch_set_tuning(whale_batch* b, const int64_t* in7);
 int whale_batch_get_tuning(const whale_batch* b, int64_t* out7);
+/* *out = 1 when batched evaluations of `mode` use the pooled launch (one
+ * launch whose compute units share their LDS as a pool of pages), else 0.
+ * That launch is switched off in this build (layout 6 is rejected), so *out
+ * is 0.  No reference counterpart (introspection). */
+int whale_batch_pooled(whale_batch* b, int32_t mode, int32_t* out);
 
 /* Memory budget (bytes) for the stored rows of ℓ, with ī and the record
  * coefficients of the reverse sweep; 0 (the default) removes it.  When the

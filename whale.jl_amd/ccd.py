@@ -217,6 +217,12 @@ class DeviceBatch:
         check(_lib.lib().whale_batch_get_tuning(self.h, ptr(o, i64p)), "whale_batch_get_tuning")
         return [int(x) for x in o]
 
+    def pooled(self, mode: str = "fp32") -> bool:
+        """Whether batched evaluations of `mode` run the pooled launch (layout 6)."""
+        n = C.c_int32()
+        check(_lib.lib().whale_batch_pooled(self.h, _lib.MODES[mode], C.byref(n)), "whale_batch_pooled")
+        return bool(n.value)
+
     def set_tuning(self, t: Sequence[int]):
         a = np.asarray(t, np.int64)
         check(_lib.lib().whale_batch_set_tuning(self.h, ptr(a, i64p)), "whale_batch_set_tuning")

@@ -98,6 +98,31 @@ __host__ __device__ inline size_t bwd_root_bytes(size_t s, size_t n, size_t nr) 
   return align16(4 * n * s) + align16(nr * s) + align16(4 * nr) + align16(2 * nr) + align16(4 * (n + 1));
 }
 
+// LDS one wave needs for family d (s bytes per value): its largest non-root
+// stage (items and records bounded separately) and its root stage, root
+// records staged or not
+__host__ __device__ inline size_t fam_lds_bytes(const FamDev& d, size_t s, bool scaled, bool fwd, bool root_recs) {
+  const size_t nr = root_recs ? d.root_recs : 0;
+  const size_t a = fwd ? fwd_stage_bytes(s, scaled, d.max_items, d.max_recs) : bwd_stage_bytes(s, d.max_items, d.max_recs);
+  const size_t b = fwd ? fwd_root_bytes(s, scaled, d.n_clades, nr) : bwd_root_bytes(s, d.n_clades, nr);
+  return a > b ? a : b;
+}
+
+// The pooled launch (k_pool): one workgroup of kPoolWaves waves per compute
+// unit; after the shared area, a header (lock and page bitmap) and a pool of
+// kPage-byte pages from which each family takes what both its directions need.
+constexpr int kPoolWaves = 12;
+constexpr uint32_t kPage = 512;
+constexpr uint32_t kPoolHdrBytes = 256;  // lock + bitmap of up to 2016 pages
+// bytes a family takes from the pool: the root records are staged when that
+// costs at most an eighth of the pool more
+__host__ __device__ inline size_t pool_need(const FamDev& d, size_t s, bool scaled, size_t pool_bytes) {
+  const size_t w0 = fam_lds_bytes(d, s, scaled, true, false), w1 = fam_lds_bytes(d, s, scaled, false, false);
+  const size_t r0 = fam_lds_bytes(d, s, scaled, true, true), r1 = fam_lds_bytes(d, s, scaled, false, true);
+  const size_t without = w0 > w1 ? w0 : w1, with = r0 > r1 ? r0 : r1;
+  return (with <= pool_bytes && with - without <= pool_bytes / 8) ? with : without;
+}
+
 // LDS of one launch of k_forward / k_backward: every wave of a workgroup runs
 // its own family in `wave_bytes` of LDS (laid out per stage); `shared_bytes`
 // (slice tables, reverse-sweep accumulators, node constants) come first and
@@ -148,6 +173,11 @@ struct FusedSlots {
 hipError_t launch_fused(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
                         const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count, int grad,
                         const FusedSlots& SL);
+// the pooled launch over families order[fam_begin .. fam_end): C.grid
+// workgroups of kPoolWaves waves, C.L.shared_bytes of shared LDS (the reverse
+// sweep's layout, tables in LDS) and C.L.wave_bytes of pool (header included)
+hipError_t launch_pool(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
+                       const ModelDev& M, const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, int grad);
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
                          double* dnode, const int32_t* hlevels, const int32_t* hnodes, int n_levels);
 hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,

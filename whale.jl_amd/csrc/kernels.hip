@@ -626,6 +626,10 @@ struct FwdRoot {
 // forward
 // ---------------------------------------------------------------------------
 constexpr int kMaxFPW = 8;  // families (waves) per workgroup, at most
+// resident waves per SIMD the DP kernels are compiled for (registers ≤ 512/3):
+// a wave mostly waits on its own LDS round trips, so a third wave per SIMD
+// hides more of them than the registers it costs
+constexpr int kWavesPerEU = 3;
 
 // The forward DP of family f by one wave: its stages (row buffers in the
 // wave's region wb), then its root.  ℓ is written through `ell` (the
@@ -1246,7 +1250,7 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
 
 
 template <typename Real, bool Scaled, bool StoreAll, bool TabL, bool Gws>
-__global__ __launch_bounds__(kWG* kMaxFPW) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+__global__ __launch_bounds__(kWG* kMaxFPW, kWavesPerEU) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                        const uint32_t* fam_count) {
   if (fam_count) fam_end = fam_begin + *fam_count;
@@ -1863,7 +1867,7 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
 }
 
 template <typename Real, bool Scaled, bool TabL, bool Gws>
-__global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+__global__ __launch_bounds__(kWG* kMaxFPW, kWavesPerEU) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                         int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
                                                         const uint32_t* fam_count) {
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
@@ -1940,7 +1944,7 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_backward(BatchDev B, ModelDev 
 // fits both directions.
 // ---------------------------------------------------------------------------
 template <typename Real, bool Scaled, bool TabL, bool Gws>
-__global__ __launch_bounds__(kWG* kMaxFPW) void k_fused(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
+__global__ __launch_bounds__(kWG* kMaxFPW, kWavesPerEU) void k_fused(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                      uint32_t fam_begin, uint32_t fam_end, const uint32_t* fam_count,
                                                      int grad, FusedSlots SL) {
   if (fam_count) fam_end = fam_begin + *fam_count;
@@ -1980,11 +1984,167 @@ __global__ __launch_bounds__(kWG* kMaxFPW) void k_fused(BatchDev B, ModelDev M, 
     wave_sync_mem();  // the slot's previous family is done with it
     fwd_family<Real, Scaled, true, TabL, Gws>(B, M, W, L, f, wb, sTab, NC, lane, ell, coef, rec0);
     if (!grad) continue;
-    if (Scaled && sizeof(Real) == sizeof(float) && W.flag[f]) continue;  // done by the FP64 pass
-    wave_sync_mem();  // the forward's rows and coefficients are visible to the wave
+    wave_sync_mem();  // the forward's rows, coefficients and flag are visible to the wave
+    if (Scaled && sizeof(Real) == sizeof(float) &&
+        __hip_atomic_load(&W.flag[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+      continue;  // done by the FP64 pass
     bwd_family<Real, Scaled, TabL, Gws>(B, M, W, L, f, wb, sTab, accPS, accX, NC, lane, ell, ibar, coef, rec0);
   }
   if (TabL && grad && !W.acc_fam) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) {
+      const Real v = accPS[j];
+      if (v != Real(0)) unsafeAtomicAdd(W.acc + j, (double)v);
+    }
+    for (int j = AL.eps + threadIdx.x; j < AL.n; j += (int)blockDim.x) {
+      const double v = accX[j];
+      if (v != 0.0) unsafeAtomicAdd(W.acc + j, v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_pool: every family's forward and (grad) reverse sweep by one wave, with
+// the family's stage buffers carved from a pool of LDS pages that the
+// workgroup's waves share.  One workgroup per compute unit holds the unit's
+// LDS: the shared area (tables, accumulators, node constants, as k_fused),
+// then the pool.  A wave takes the next family, allocates the pages that
+// family needs (contiguous, first fit), runs it and frees them, so small
+// families run many to a compute unit and large ones few, in one launch over
+// the whole batch and with no boundary between the directions.  Rows, ī and
+// coefficients go to the batch's full storage as in the stored form.
+//
+// Allocation: the wave holding the pool lock takes the next ticket and waits
+// (lock held) until its pages are free; frees need no lock (atomic and-not on
+// the page bitmap), and a wave holding pages never waits for another wave,
+// so every allocation of a family no larger than the empty pool completes.
+// The waits are bounded all the same: a family that cannot be placed within
+// the bound gets a NaN logL instead of a hang.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPoolSpin = 1u << 21;  // sleeps before an allocation gives up
+
+struct PoolHdr {
+  uint32_t lock;
+  uint32_t bits[kPoolHdrBytes / 4 - 1];  // page bitmap, 1 = in use
+};
+
+// lane 0: the lock, then the next ticket and its pages.  Returns the ticket
+// (≥ n_tickets when the batch is done) and the first page (kNone if the wait
+// gave up).
+__device__ inline uint32_t pool_take(PoolHdr* H, uint32_t npages, uint32_t* ticket, uint32_t n_tickets,
+                                     const BatchDev& B, uint32_t fam_begin, size_t s, bool scaled, uint32_t pool_bytes,
+                                     uint32_t* first, uint32_t* pages) {
+  uint32_t spin = 0;
+  for (;;) {
+    uint32_t expect = 0u;
+    if (__hip_atomic_compare_exchange_strong(&H->lock, &expect, 1u, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      break;
+    __builtin_amdgcn_s_sleep(4);
+    if (++spin > kPoolSpin) {  // never expected: the lock holder's wait is bounded too
+      *first = kNone;
+      *pages = 0;
+      return atomicAdd(ticket, 1u);
+    }
+  }
+  const uint32_t k = atomicAdd(ticket, 1u);
+  *first = kNone;
+  *pages = 0;
+  if (k < n_tickets) {
+    const FamDev& d = B.fam[B.order[fam_begin + k]];
+    const uint32_t need = (uint32_t)pool_need(d, s, scaled, pool_bytes);
+    const uint32_t np = (need + kPage - 1) / kPage;
+    *pages = np;
+    for (spin = 0; spin < kPoolSpin; ++spin) {
+      uint32_t run = 0;
+      for (uint32_t p = 0; p < npages; ++p) {
+        const uint32_t used = (__hip_atomic_load(&H->bits[p >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >>
+                               (p & 31)) & 1u;
+        run = used ? 0u : run + 1u;
+        if (run == np) {
+          *first = p + 1 - np;
+          break;
+        }
+      }
+      if (*first != kNone) break;
+      __builtin_amdgcn_s_sleep(8);
+    }
+    if (*first != kNone)
+      for (uint32_t p = *first; p < *first + np;) {  // mark, a word at a time
+        const uint32_t w = p >> 5, lo = p & 31, hi = min(32u, lo + (*first + np - p));
+        const uint32_t m = (hi == 32 ? ~0u : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+        __hip_atomic_fetch_or(&H->bits[w], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        p += hi - lo;
+      }
+  }
+  __hip_atomic_store(&H->lock, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+  return k;
+}
+
+__device__ inline void pool_free(PoolHdr* H, uint32_t first, uint32_t np) {
+  for (uint32_t p = first; p < first + np;) {
+    const uint32_t w = p >> 5, lo = p & 31, hi = min(32u, lo + (first + np - p));
+    const uint32_t m = (hi == 32 ? ~0u : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+    __hip_atomic_fetch_and(&H->bits[w], ~m, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    p += hi - lo;
+  }
+}
+
+template <typename Real, bool Scaled, bool StoreAll>
+__global__ __launch_bounds__(kWG* kPoolWaves, kWavesPerEU) void k_pool(BatchDev B, ModelDev M, WorkDev<Real> W,
+                                                                     LdsCfg L, uint32_t fam_begin, uint32_t fam_end,
+                                                                     int grad) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t lane = threadIdx.x % kWG;
+  const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
+  const int T2 = 2 * M.tab_rows;
+  Real* sTab = reinterpret_cast<Real*>(smem);
+  const size_t tab_b = align16((size_t)T2 * sizeof(Real));
+  Real* const accPS = reinterpret_cast<Real*>(smem + tab_b);
+  double* const accX = reinterpret_cast<double*>(smem + 2 * tab_b) - AL.eps;
+  NodeC<Real> NC(smem + 2 * tab_b + align16(8 * (size_t)(AL.n - AL.eps)), M.n);
+  PoolHdr* const H = reinterpret_cast<PoolHdr*>(smem + L.shared_bytes);
+  unsigned char* const pool = smem + L.shared_bytes + kPoolHdrBytes;
+  const uint32_t pool_bytes = L.wave_bytes - kPoolHdrBytes;
+  const uint32_t npages = pool_bytes / kPage;
+  for (int r = threadIdx.x; r < M.tab_rows; r += (int)blockDim.x) {
+    sTab[2 * r] = (Real)M.phi[r];
+    sTab[2 * r + 1] = (Real)M.psi[r];
+  }
+  for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) accPS[j] = 0;
+  for (int j = AL.eps + threadIdx.x; j < AL.n; j += (int)blockDim.x) accX[j] = 0.0;
+  for (int j = threadIdx.x; j < (int)(kPoolHdrBytes / 4); j += (int)blockDim.x) reinterpret_cast<uint32_t*>(H)[j] = 0u;
+  NC.fill(M, threadIdx.x, (int)blockDim.x);
+  __syncthreads();
+  const uint32_t n_tickets = fam_end - fam_begin;
+  for (;;) {
+    uint32_t k = 0, first = 0, np = 0;
+    if (lane == 0)
+      k = pool_take(H, npages, W.ticket, n_tickets, B, fam_begin, sizeof(Real), Scaled, pool_bytes, &first, &np);
+    k = __builtin_amdgcn_readfirstlane(k);
+    if (k >= n_tickets) break;
+    first = __builtin_amdgcn_readfirstlane(first);
+    np = __builtin_amdgcn_readfirstlane(np);
+    const uint32_t f = __builtin_amdgcn_readfirstlane(B.order[fam_begin + k]);
+    if (first == kNone) {  // the allocation gave up (not expected): the result says so
+      if (lane == 0) W.fam_ll[f] = __builtin_nan("");
+      continue;
+    }
+    const LdsCfg Lf{np * kPage, L.shared_bytes};
+    unsigned char* const wb = pool + (size_t)first * kPage;
+    fwd_family<Real, Scaled, StoreAll, true, false>(B, M, W, Lf, f, wb, sTab, NC, lane, W.ell + B.fam[f].ell0, W.coef,
+                                                    0u);
+    wave_sync_mem();  // the forward's rows, coefficients and flag are visible to the wave
+    const bool flagged = Scaled && sizeof(Real) == sizeof(float) &&
+                         __hip_atomic_load(&W.flag[f], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+    if (grad && !flagged) {  // else done by the FP64 pass
+      bwd_family<Real, Scaled, true, false>(B, M, W, Lf, f, wb, sTab, accPS, accX, NC, lane, W.ell + B.fam[f].ell0,
+                                            W.ibar + B.fam[f].ibar0, Scaled ? W.coef : nullptr, 0u);
+    }
+    wave_sync();  // the wave's last reads and writes of its pages are done
+    if (lane == 0) pool_free(H, first, np);
+  }
+  if (grad && !W.acc_fam) {
     __syncthreads();
     for (int j = threadIdx.x; j < T2; j += (int)blockDim.x) {
       const Real v = accPS[j];
@@ -2253,6 +2413,24 @@ hipError_t launch_fused(int mode, const LaunchCfg& C, hipStream_t st, const Batc
   if (mode == WHALE_FP32_SCALED)
     return fused_variant<float, true>(C, st, B, M, W, fam_begin, fam_end, fam_count, grad, SL);
   return fused_variant<double, true>(C, st, B, M, W, fam_begin, fam_end, fam_count, grad, SL);
+}
+
+template <typename R, bool S>
+static hipError_t pool_variant(bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
+                               const ModelDev& M, const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, int grad) {
+  const WorkDev<R> w(W);
+  if (C.L.wave_bytes <= kPoolHdrBytes || (grad && !store_all)) return hipErrorInvalidValue;
+  const size_t lds = (size_t)C.L.shared_bytes + C.L.wave_bytes;
+  if (store_all)
+    return go(k_pool<R, S, true>, C.grid, kPoolWaves, lds, st, B, M, w, C.L, fam_begin, fam_end, grad);
+  return go(k_pool<R, S, false>, C.grid, kPoolWaves, lds, st, B, M, w, C.L, fam_begin, fam_end, grad);
+}
+
+hipError_t launch_pool(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
+                       const ModelDev& M, const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, int grad) {
+  if (mode == WHALE_FP64_PLAIN) return pool_variant<double, false>(store_all, C, st, B, M, W, fam_begin, fam_end, grad);
+  if (mode == WHALE_FP32_SCALED) return pool_variant<float, true>(store_all, C, st, B, M, W, fam_begin, fam_end, grad);
+  return pool_variant<double, true>(store_all, C, st, B, M, W, fam_begin, fam_end, grad);
 }
 
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
