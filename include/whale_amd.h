@@ -191,6 +191,17 @@ int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t
  * >= 1024), layout 0..5, waves per CU 1..32].  No reference counterpart. */
 int whale_batch_set_tuning(whale_batch* b, const int64_t* in7);
 int whale_batch_get_tuning(const whale_batch* b, int64_t* out7);
+/* Per-family weights of the reverse sweep for the following gradient calls
+ * (n_families finite values >= 0; NULL restores weight 1): the gradient
+ * becomes that of Σ_f seeds[f]·(logL_f − c), c the conditioning term, i.e.
+ * each family's derivative is scaled by its weight and the conditioning
+ * term's by the weights' sum.  The value and per-family results are
+ * unchanged.  No reference counterpart: the reference differentiates
+ * logpdf(MixtureModel, xs) (src/core.jl:66-76) with ForwardDiff, whose
+ * derivative for component j is this gradient with the families'
+ * responsibilities for j as weights. */
+int whale_batch_set_seeds(whale_batch* b, const double* seeds);
+
 /* *out = 1 when batched evaluations of `mode` use the pooled launch (one
  * launch whose compute units share their LDS as a pool of pages), else 0.
  * That launch is switched off in this build (layout 6 is rejected), so *out

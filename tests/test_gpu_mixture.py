@@ -106,3 +106,66 @@ def test_nowhere_extinct_batch_value_and_gradient(w):
     err = np.max(np.abs(g - fd)) / np.max(np.abs(fd))
     print("NowhereExtinct gradient vs central differences: rel err", err)
     assert err <= 1e-5
+
+
+def test_family_weights_select_families(w, cfg2):
+    """The reverse sweep's family weights: all ones give the plain gradient;
+    a weight vector e_k gives the gradient of family k alone (conditioning
+    included once); weights scale linearly."""
+    wl, om, oc, m, c, comps = cfg2
+    n = len(c)
+    _, g64, _ = w.gradient(m, c, mode="fp64")
+    for mode, tol in (("fp64", 1e-10), ("fp32", 1e-5)):
+        v, g, fam = w.gradient(m, c, mode=mode)
+        v1, g1, fam1 = w.gradient(m, c, mode=mode, weights=np.ones(n))
+        assert v1 == v and np.array_equal(fam1, fam)
+        # (FP32 accumulators are summed in a run-dependent order)
+        assert np.max(np.abs(g1 - g)) <= (1e-12 if mode == "fp64" else 1e-6) * np.max(np.abs(g))
+        for k in (0, 7, n - 1):
+            e = np.zeros(n)
+            e[k] = 1.0
+            _, gk, _ = w.gradient(m, c, mode=mode, weights=e)
+            _, gs, _ = w.gradient(m, w.ccds_from_texts([wl.texts[k]], m), mode=mode)
+            err = np.max(np.abs(gk - gs)) / np.max(np.abs(gs))
+            print(mode, "family", k, "weighted vs alone", err)
+            assert err <= tol
+        wt = np.linspace(0.0, 2.0, n)
+        _, ga, _ = w.gradient(m, c, mode=mode, weights=wt)
+        _, gb, _ = w.gradient(m, c, mode=mode, weights=3.0 * wt)
+        assert np.max(np.abs(gb - 3.0 * ga)) <= 1e-6 * np.max(np.abs(gb))
+    # weights never change later unweighted calls
+    _, g2, _ = w.gradient(m, c, mode="fp64")
+    assert np.max(np.abs(g2 - g64)) <= 1e-12 * np.max(np.abs(g64))
+
+
+def test_mixture_gradient_against_oracle(w, cfg2):
+    """d logpdf(MixtureModel)/dθ_j = Σ_i r_ij ∂(ℓ_ij − c_j) from per-family
+    dual-number oracle gradients, and d/dprior_j = Σ_i r_ij / prior_j against
+    central differences of the mixture value."""
+    wl, om, oc, m, c, comps = cfg2
+    nsub = 16
+    texts = wl.texts[:nsub]
+    cs = w.ccds_from_texts(texts, m)
+    prior = [0.5, 0.3, 0.2]
+    mm = w.MixtureModel([m(r) for r in comps], prior)
+    tot, R = mm.responsibilities(cs)
+    assert np.allclose(R.sum(axis=1), 1.0)
+    for mode, tol in (("fp64", 1e-9), ("fp32", 1e-5)):
+        v, grads, dprior = mm.gradient(cs, mode=mode)
+        assert abs(v - tot) <= (1e-12 if mode == "fp64" else 1e-6) * abs(tot)
+        for j, r in enumerate(comps):
+            ref = np.zeros_like(grads[j])
+            for i, t in enumerate(texts):
+                _, gi = om.gradient(om.ccds([t]), orates(r))
+                ref += R[i, j] * gi
+            err = np.max(np.abs(grads[j] - ref)) / np.max(np.abs(ref))
+            print(mode, "component", j, "mixture gradient rel err", err)
+            assert err <= tol
+    h = 1e-6
+    for j in range(3):
+        pp, pm = list(prior), list(prior)
+        pp[j] += h
+        pm[j] -= h
+        fd = (w.MixtureModel(mm.components, pp).logpdf(cs) - w.MixtureModel(mm.components, pm).logpdf(cs)) / (2 * h)
+        print("prior", j, dprior[j], fd)
+        assert abs(dprior[j] - fd) <= 1e-6 * max(1.0, abs(fd))

@@ -217,6 +217,17 @@ class DeviceBatch:
         check(_lib.lib().whale_batch_get_tuning(self.h, ptr(o, i64p)), "whale_batch_get_tuning")
         return [int(x) for x in o]
 
+    def set_seeds(self, seeds=None):
+        """Per-family weights of the reverse sweep for later gradient calls
+        (whale_batch_set_seeds); None restores 1."""
+        if seeds is None:
+            check(_lib.lib().whale_batch_set_seeds(self.h, None), "whale_batch_set_seeds")
+            return
+        a = np.ascontiguousarray(seeds, dtype=np.float64)
+        if a.shape != (self.n,):
+            raise ValueError(f"one seed per family ({self.n})")
+        check(_lib.lib().whale_batch_set_seeds(self.h, ptr(a, dblp)), "whale_batch_set_seeds")
+
     def pooled(self, mode: str = "fp32") -> bool:
         """Whether batched evaluations of `mode` run the pooled launch (layout 6)."""
         n = C.c_int32()

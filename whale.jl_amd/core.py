@@ -105,10 +105,13 @@ def logpdf_families(wm, ccds, mode: str = "fp64", launch: str = "batched") -> np
     return _eval(wm, ccds, mode, launch, False)[1]
 
 
-def gradient(wm, ccds, mode: str = "fp64", launch: str = "batched"):
+def gradient(wm, ccds, mode: str = "fp64", launch: str = "batched", weights=None):
     """(value, gradient, per-family values): the conditioned batch
     log-likelihood and its derivative with respect to the rates' parameter
-    vector (DLWGD: [λ; μ; q; η], log-scale λ/μ; ConstantDLWGD: [λ, μ, q; η])."""
+    vector (DLWGD: [λ; μ; q; η], log-scale λ/μ; ConstantDLWGD: [λ, μ, q; η]).
+    With `weights` (one per family, ≥ 0) the gradient is that of
+    Σ_i weights[i]·(ℓ_i − c) instead (the reverse sweep's family weights,
+    whale_batch_set_seeds); the value and per-family values are unchanged."""
     if isinstance(ccds, CCD):
         ccds = _single_set(ccds)
     host = getattr(wm.s.condition, "host", False)
@@ -124,21 +127,29 @@ def gradient(wm, ccds, mode: str = "fp64", launch: str = "batched"):
     dq = np.zeros(max(1, nw))
     deta = C.c_double()
     tot = C.c_double()
-    check(_lib.lib().whale_logpdf_grad(b.h, _mode(mode), _launch(launch), ptr(fam, dblp), C.byref(tot),
-                                       ptr(dl, dblp), ptr(dm, dblp), ptr(dq, dblp), C.byref(deta)),
-          "whale_logpdf_grad")
+    if weights is not None:
+        b.set_seeds(weights)
+    try:
+        check(_lib.lib().whale_logpdf_grad(b.h, _mode(mode), _launch(launch), ptr(fam, dblp), C.byref(tot),
+                                           ptr(dl, dblp), ptr(dm, dblp), ptr(dq, dblp), C.byref(deta)),
+              "whale_logpdf_grad")
+    finally:
+        if weights is not None:
+            b.set_seeds(None)
     total, dt = tot.value, deta.value
     if host:
         # a condition applied on the host: the device returned Σ ℓ_i and its
-        # gradient; subtract N·c and N·∂c (natural scale, per node)
+        # gradient; subtract N·c and N·∂c (natural scale, per node), N the
+        # families' total weight for the gradient
         from .mixture import nowhere_extinct, nowhere_extinct_grad
         nf = len(fam)
+        ng = nf if weights is None else float(np.sum(weights))
         cl, cm, cq, ce = nowhere_extinct_grad(wm)
         total -= nf * nowhere_extinct(wm)
-        dl -= nf * cl
-        dm -= nf * cm
-        dq[:len(cq)] -= nf * cq
-        dt -= nf * ce
+        dl -= ng * cl
+        dm -= ng * cm
+        dq[:len(cq)] -= ng * cq
+        dt -= ng * ce
     g = wm.chain_gradient(dl, dm, dq, dt)
     return total, g, fam
 

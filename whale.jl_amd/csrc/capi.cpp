@@ -212,6 +212,10 @@ struct whale_batch {
   // repeated evaluations give bit-identical gradients.
   bool det = false;
   DBuf acc_fam;
+  // Per-family weights of the reverse sweep (whale_batch_set_seeds): n_fam
+  // seeds, then their sum; used by gradient calls while `seeded`.
+  bool seeded = false;
+  DBuf seed;
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   // the pooled launch per mode (layout kLayoutPool): 0 not planned, 1 planned
@@ -1224,6 +1228,7 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   const ModelDev M = m->dev();
   WorkRaw W = ws ? *ws : b->work();
   W.acc_fam = b->det && grad ? b->acc_fam.as<double>() : nullptr;
+  W.seed = b->seeded && grad ? b->seed.as<double>() : nullptr;
   const AccLayout AL(m->h.tab_rows, m->h.n, m->h.n_wgd);
   if (timed) HIPCHK(hipEventRecord(c->ev[1], st));
   HIPCHK(launch_fill_u32(st, W.ticket, 0u, kTickets));
@@ -1461,7 +1466,8 @@ int enqueue(whale_batch* b, int mode, int32_t launch, bool grad, bool store_all,
   if (W.acc_fam) HIPCHK(launch_det_sum(st, W.acc_fam, nf, AL.n, W.acc));
   double* out = ws_out ? ws_out : m->out.as<double>();
   HIPCHK(launch_tail(st, M, W.fam_ll, nf, W.acc, m->ebar.as<double>(), m->drow.as<double>(), m->dnode.as<double>(),
-                     m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out));
+                     m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, m->h.cond, grad ? 1 : 0, out,
+                     W.seed ? W.seed + nf : nullptr));
   if (timed) HIPCHK(hipEventRecord(c->ev[4], st));
   if (ws) return WHALE_OK;
   const int n = m->h.n, nw = m->h.n_wgd;
@@ -1754,6 +1760,29 @@ int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t
     out_ms2[0] = t_start;
     out_ms2[1] = t_best;
   }
+  return WHALE_OK;
+}
+
+int whale_batch_set_seeds(whale_batch* b, const double* seeds) {
+  if (!b) return fail(WHALE_EINVAL, "NULL batch");
+  HIPCHK(hipSetDevice(b->model->ctx->device));
+  if (!seeds) {
+    if (b->seeded) drop_graphs(b);  // captured evaluations hold the seed pointer
+    b->seeded = false;
+    return WHALE_OK;
+  }
+  std::vector<double> h(seeds, seeds + b->n_fam);
+  double sum = 0;
+  for (double x : h) {
+    if (!std::isfinite(x) || x < 0) return fail(WHALE_EINVAL, "seeds must be finite and non-negative");
+    sum += x;
+  }
+  h.push_back(sum);
+  const void* before = b->seed.p;
+  HIPCHK(b->seed.alloc(sizeof(double) * h.size()));
+  HIPCHK(hipMemcpy(b->seed.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  if (!b->seeded || b->seed.p != before) drop_graphs(b);
+  b->seeded = true;
   return WHALE_OK;
 }
 

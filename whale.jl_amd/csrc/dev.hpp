@@ -54,6 +54,10 @@ struct WorkRaw {
   uint32_t* fb;       // fb[0] = number of such families, fb[1..] = their indices
   unsigned char* gws; // stage buffers of launches whose families do not fit LDS (LaunchCfg::gws)
   double* acc_fam = nullptr;  // deterministic gradients: per-family accumulator slots
+  // per-family weights of the reverse sweep (null: 1): family f's adjoint is
+  // seeded with seed[f] / L_f, and seed[n_fam] = Σ seed scales the
+  // conditioning term's derivative (mixture responsibilities)
+  const double* seed = nullptr;
 };
 
 // gradient accumulator layout (doubles)
@@ -180,9 +184,12 @@ hipError_t launch_pool(int mode, bool store_all, const LaunchCfg& C, hipStream_t
                        const ModelDev& M, const WorkRaw& W, uint32_t fam_begin, uint32_t fam_end, int grad);
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,
                          double* dnode, const int32_t* hlevels, const int32_t* hnodes, int n_levels);
+// n_eff (null: n_fam) holds the family count the conditioning term's
+// derivative is scaled by (Σ of the reverse sweep's family weights)
 hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,
                        double* ebar, const double* drow, const double* dnode, const int32_t* dlevels,
-                       const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out);
+                       const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out,
+                       const double* n_eff = nullptr);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t v, int n);
 hipError_t launch_det_sum(hipStream_t st, const double* slots, uint32_t n_fam, int ncol, double* out);
 hipError_t launch_fill_f64(hipStream_t st, double* p, double v, size_t n);

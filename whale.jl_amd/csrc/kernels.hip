@@ -68,9 +68,10 @@ struct WorkDev {
   uint32_t* fb;
   unsigned char* gws;
   double* acc_fam;    // deterministic gradients: per-family accumulator slots (null otherwise)
+  const double* seed; // per-family weights of the reverse sweep (null: 1)
   explicit WorkDev(const WorkRaw& w)
       : ell((Real*)w.ell), coef((Real*)w.coef), E(w.E), ibar((Real*)w.ibar), fam_ll(w.fam_ll), acc(w.acc),
-        ticket(w.ticket), flag(w.flag), fb(w.fb), gws(w.gws), acc_fam(w.acc_fam) {}
+        ticket(w.ticket), flag(w.flag), fb(w.fb), gws(w.gws), acc_fam(w.acc_fam), seed(w.seed) {}
 };
 
 __device__ __forceinline__ double eps_top(const ModelDev& M, int node) {
@@ -1435,7 +1436,7 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
     const Real Lr = bV[n - 1];
     const bool ok = Lr > Real(0) && isfinite((double)Lr);
     if (ok) {
-      if (lane == 0) bA[n - 1] = Real(1) / Lr;
+      if (lane == 0) bA[n - 1] = (W.seed ? (Real)W.seed[f] : Real(1)) / Lr;
       wsync<Gws>();
       // d logL / d ℓ of each root clade, larger clades first, scattered into
       // the parts; a level's records (those of its clades) one per lane
@@ -2172,7 +2173,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(ModelDev M, const double*
                                                        const double* acc_in, double* ebar, const double* drow,
                                                        const double* dnode, const int32_t* dlevels,
                                                        const int32_t* dnodes, int n_dlevels, int cond, int with_grad,
-                                                       double* out) {
+                                                       double* out, const double* n_eff) {
   __shared__ double red[kTailThreads];
   __shared__ double s_total, s_eta;
   const int t = threadIdx.x;
@@ -2233,7 +2234,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(ModelDev M, const double*
   }
   const double N = (double)n_fam;
   if (t == 0) {
-    double tot = red[0] - N * c;
+    const double tot = red[0] - N * c;
     s_total = isfinite(tot) ? tot : -INFINITY;
   }
   if (!with_grad) {
@@ -2250,10 +2251,11 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(ModelDev M, const double*
     const double dA_der = -(1.0 - eta) * (1.0 - eta) / eta;
     const double dB_deta = (1.0 - er) / (xi * xi) - 2.0 * eta * (1.0 - er) * er / (xi * xi * xi);
     const double dB_der = -eta / (xi * xi) + 2.0 * eta * (1.0 - er) * (1.0 - eta) / (xi * xi * xi);
-    ebar[M.root] += A_ * dA_der + B_ * dB_der - N * dc_er;
-    ebar[rn.child0] += -N * dc_ef;
-    ebar[rn.child1] += -N * dc_eg;
-    s_eta = A_ * dA_deta + B_ * dB_deta - N * dc_eta;
+    const double Ng = n_eff ? *n_eff : N;  // the families' weight in the gradient
+    ebar[M.root] += A_ * dA_der + B_ * dB_der - Ng * dc_er;
+    ebar[rn.child0] += -Ng * dc_ef;
+    ebar[rn.child1] += -Ng * dc_eg;
+    s_eta = A_ * dA_deta + B_ * dB_deta - Ng * dc_eta;
   }
   __syncthreads();
   double* dl = out + 1;
@@ -2442,9 +2444,10 @@ hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double*
 
 hipError_t launch_tail(hipStream_t st, const ModelDev& M, const double* fam_ll, uint32_t n_fam, const double* acc,
                        double* ebar, const double* drow, const double* dnode, const int32_t* dlevels,
-                       const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out) {
+                       const int32_t* dnodes, int n_dlevels, int cond, int with_grad, double* out,
+                       const double* n_eff) {
   hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailThreads), 0, st, M, fam_ll, n_fam, acc, ebar, drow, dnode, dlevels, dnodes,
-                     n_dlevels, cond, with_grad, out);
+                     n_dlevels, cond, with_grad, out, n_eff);
   return hipGetLastError();
 }
 

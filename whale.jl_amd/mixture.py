@@ -17,7 +17,7 @@ from typing import Sequence
 import numpy as np
 
 from .ccd import CCD
-from .core import logpdf, logpdf_families
+from .core import gradient, logpdf, logpdf_families
 from .model import NoCondition, NonExtinctCondition, NowhereExtinctCondition, RootCondition
 
 
@@ -157,6 +157,29 @@ class MixtureModel:
     def logpdf(self, ccds, mode: str = "fp64") -> float:
         tot = float(np.sum(_logsumexp_rows(self.family_matrix(ccds, mode))))
         return tot if math.isfinite(tot) else -math.inf
+
+    def responsibilities(self, ccds, mode: str = "fp64"):
+        """(logpdf, r): r[i, j] the posterior probability that family i comes
+        from component j (rows of families no component can produce are 0)."""
+        F = self.family_matrix(ccds, mode)
+        lse = _logsumexp_rows(F)
+        R = np.zeros_like(F)
+        ok = np.isfinite(lse)
+        R[ok] = np.exp(F[ok] - lse[ok, None])
+        tot = float(np.sum(lse))
+        return (tot if math.isfinite(tot) else -math.inf), R
+
+    def gradient(self, ccds, mode: str = "fp64"):
+        """(value, [gradient of each component], d/dprior): the derivative of
+        logpdf(MixtureModel, ccds) (src/core.jl:66-76) with respect to each
+        component's rates vector (as `gradient` lays it out) and to the prior
+        weights, i.e. Σ_i r_ij ∂(ℓ_ij − c_j) and Σ_i r_ij / prior[j].  One
+        device evaluation per component gives ℓ_ij, one reverse sweep per
+        component with the responsibilities r_ij as the families' weights
+        gives the rest."""
+        tot, R = self.responsibilities(ccds, mode)
+        grads = [gradient(wm, ccds, mode=mode, weights=R[:, j])[1] for j, wm in enumerate(self.components)]
+        return tot, grads, R.sum(axis=0) / self.prior
 
 
 class ModelArray:
