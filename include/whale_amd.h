@@ -202,6 +202,26 @@ int whale_batch_get_tuning(const whale_batch* b, int64_t* out7);
  * responsibilities for j as weights. */
 int whale_batch_set_seeds(whale_batch* b, const double* seeds);
 
+/* logpdf of a finite mixture of models over the batch's families, and its
+ * gradient (src/core.jl:66-76: Σ_i logsumexp_j (ℓ_ij + log prior_j − c_j)).
+ * The K components (1..8) are models with the batch's species-tree structure
+ * and their own parameters (set); every size-class launch evaluates all K
+ * components (its grid K times the single model's, each workgroup working
+ * for one component with that component's tables and workspace), a mixture
+ * kernel forms the log-sum-exp and the responsibilities r_ij on the device,
+ * and with `gradient` the reverse sweeps of all components run as one launch
+ * per class with r_ij as the families' weights.  Not with deterministic
+ * gradients, an ℓ budget in force, or families whose stages do not fit LDS.
+ *   out_total: the mixture log-likelihood
+ *   out_fam (n_families × K, may be NULL): unconditioned ℓ_ij, family-major
+ *   out_resp (n_families × K, may be NULL): r_ij
+ *   out_grad (K × (2n + n_wgd + 1), with gradient): per component
+ *            [dλ (n), dμ (n), dq (n_wgd), dη] on the natural scale
+ *   out_dprior (K, may be NULL): d/dprior_j = Σ_i r_ij / prior_j */
+int whale_mixture_eval(whale_batch* b, whale_model* const* comps, int32_t K, const double* prior, int32_t mode,
+                       int32_t gradient, double* out_total, double* out_fam, double* out_resp, double* out_grad,
+                       double* out_dprior);
+
 /* *out = 1 when batched evaluations of `mode` use the pooled launch (one
  * launch whose compute units share their LDS as a pool of pages), else 0.
  * That launch is switched off in this build (layout 6 is rejected), so *out

@@ -169,3 +169,63 @@ def test_mixture_gradient_against_oracle(w, cfg2):
         fd = (w.MixtureModel(mm.components, pp).logpdf(cs) - w.MixtureModel(mm.components, pm).logpdf(cs)) / (2 * h)
         print("prior", j, dprior[j], fd)
         assert abs(dprior[j] - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_device_mixture_matches_per_component_path(w, cfg2):
+    """whale_mixture_eval (all components in each size-class launch, the
+    log-sum-exp and responsibilities on the device, one weighted reverse
+    launch per class) against the per-component path (one evaluation per
+    component combined on the host, one weighted reverse sweep each)."""
+    wl, om, oc, m, c, comps = cfg2
+    prior = [0.5, 0.3, 0.2]
+    mm = w.MixtureModel([m(r) for r in comps], prior)
+    assert mm.on_device(c)
+    for mode, vtol, gtol in (("fp64", 1e-12, 1e-10), ("fp32", 1e-6, 1e-5)):
+        F = mm.family_matrix(c, mode)
+        tot_h, R_h = mm._host_resp(c, mode)
+        tot, fam, R, g, dprior = mm._device_eval(c, mode, True)
+        fam_ref = np.stack([w.logpdf_families(wm, c, mode=mode) for wm in mm.components], axis=1)
+        assert np.max(np.abs(fam - fam_ref) / np.abs(fam_ref)) <= (1e-13 if mode == "fp64" else 1e-6)
+        assert abs(tot - tot_h) <= vtol * abs(tot_h)
+        assert np.max(np.abs(R - R_h)) <= (1e-10 if mode == "fp64" else 1e-5)
+        v, grads, dp = mm.gradient(c, mode=mode)
+        vh, grads_h, dp_h = mm.gradient(c, mode=mode, device=False)
+        for j in range(3):
+            err = np.max(np.abs(grads[j] - grads_h[j])) / np.max(np.abs(grads_h[j]))
+            print(mode, "component", j, "device vs per-component gradient", err)
+            assert err <= gtol
+        assert np.max(np.abs(dp - dp_h) / np.abs(dp_h)) <= (1e-10 if mode == "fp64" else 1e-5)
+    # one component with weight one is the model itself
+    one = w.MixtureModel([m(comps[0])], [1.0])
+    v1, g1, _ = one.gradient(c, mode="fp64")
+    v0, g0, _ = w.gradient(m(comps[0]), c, mode="fp64")
+    assert abs(v1 - v0) <= 1e-12 * abs(v0)
+    assert np.max(np.abs(g1[0] - g0)) <= 1e-10 * np.max(np.abs(g0))
+
+
+def test_device_mixture_fp32_overflow_and_routed(w):
+    """Components on a batch holding the family FP32 cannot hold: the first
+    mixture call recomputes it per component in FP64 (each component's own
+    flagged list); after a single-model FP32 call has routed it to FP64, the
+    mixture's routed launch covers it for every component."""
+    from ale_util import ale_from_tree, balanced_paralogs
+    tree, dt = "((A:0.03,B:1.0):1.0,C:1.03);", 0.005
+    lam, mu = [np.log(0.8)] * 5, [np.log(0.3)] * 5
+    mu[1] = np.log(3.0)
+    big = ale_from_tree(*balanced_paralogs(64, "A", ("B_1", "C_1")))
+    small = [ale_from_tree(*balanced_paralogs(k, "A", ("B_1", "C_1"))) for k in (1, 2, 3, 5, 8)]
+    texts = small[:2] + [big] + small[2:] + [big]
+    m = w.WhaleModel(w.DLWGD(λ=lam, μ=mu, η=0.9), w.readnw(tree), dt)
+    c = w.ccds_from_texts(texts, m)
+    lam2 = [x + 0.05 for x in lam]
+    mm = w.MixtureModel([m, m(w.DLWGD(λ=lam2, μ=mu, η=0.85))], [0.6, 0.4])
+    for call in range(2):
+        v64, g64, d64 = mm.gradient(c, mode="fp64")
+        v32, g32, d32 = mm.gradient(c, mode="fp32")
+        print("call", call, v64, v32)
+        assert abs(v32 - v64) <= 1e-6 * abs(v64)
+        for a, b in zip(g32, g64):
+            assert np.max(np.abs(a - b)) <= 1e-5 * np.max(np.abs(b))
+        assert np.max(np.abs(d32 - d64) / np.abs(d64)) <= 1e-5
+        w.gradient(m, c, mode="fp32")  # flags the overflowing families: later calls route them to FP64
+    assert w.device_batch(m, c).fallbacks() == 2

@@ -74,6 +74,8 @@ def lib():
         L.whale_batch_get_tuning.argtypes = [vp, i64p]
         L.whale_batch_pooled.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32)]
         L.whale_batch_set_seeds.argtypes = [vp, dblp]
+        L.whale_mixture_eval.argtypes = [vp, C.POINTER(vp), C.c_int32, dblp, C.c_int32, C.c_int32, dblp, dblp,
+                                         dblp, dblp, dblp]
         L.whale_batch_launches.argtypes = [vp, C.c_int32, i64p]
         L.whale_logpdf.argtypes = [vp, C.c_int32, C.c_int32, dblp, dblp]
         L.whale_logpdf_grad.argtypes = [vp, C.c_int32, C.c_int32, dblp, dblp, dblp, dblp, dblp, dblp]

@@ -216,6 +216,14 @@ struct whale_batch {
   // seeds, then their sum; used by gradient calls while `seeded`.
   bool seeded = false;
   DBuf seed;
+  // Mixture evaluations (whale_mixture_eval): one workspace per component,
+  // the per-launch component tables, the log priors, the result and the
+  // responsibilities.
+  struct MixWork {
+    DBuf ell, coef, E, ibar, fam_ll, acc, ticket, flag, fb, seed, out;
+  };
+  std::vector<std::unique_ptr<MixWork>> mixw;
+  DBuf mixsel, mixprior, mixout, mixresp;
   LaunchCfg cfg[3][2][kClasses];   // [mode][forward, backward][class]
   LaunchCfg cfg_fb[2];             // FP64 pass over families the FP32 form could not represent
   // the pooled launch per mode (layout kLayoutPool): 0 not planned, 1 planned
@@ -1759,6 +1767,226 @@ int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t
   if (out_ms2) {
     out_ms2[0] = t_start;
     out_ms2[1] = t_best;
+  }
+  return WHALE_OK;
+}
+
+int whale_mixture_eval(whale_batch* b, whale_model* const* comps, int32_t K, const double* prior, int32_t mode,
+                       int32_t gradient, double* out_total, double* out_fam, double* out_resp, double* out_grad,
+                       double* out_dprior) {
+  if (!b || !comps || !prior || !out_total) return fail(WHALE_EINVAL, "NULL argument");
+  if (K < 1 || K > kMaxMix) return fail(WHALE_EINVAL, "1 to 8 mixture components");
+  if (mode < 0 || mode > 2) return fail(WHALE_EINVAL, "unknown mode");
+  if (gradient && !out_grad) return fail(WHALE_EINVAL, "out_grad is NULL");
+  if (b->det || use_fused(b, WHALE_LAUNCH_BATCHED, false))
+    return fail(WHALE_ESTATE, "mixtures need the stored form without deterministic gradients");
+  if (b->hbm_hi > b->hbm_lo) return fail(WHALE_ESTATE, "mixtures need every family's stages in LDS");
+  whale_model* m0 = b->model;
+  const ModelHost& H = m0->h;
+  for (int j = 0; j < K; ++j) {
+    const whale_model* m = comps[j];
+    if (!m) return fail(WHALE_EINVAL, "NULL component");
+    if (!(prior[j] > 0) || !std::isfinite(prior[j])) return fail(WHALE_EINVAL, "prior weights must be positive");
+    if (!m->has_params) return fail(WHALE_ESTATE, "component parameters not set (whale_model_set_params)");
+    bool same = m->ctx == m0->ctx && m->h.n == H.n && m->h.n_wgd == H.n_wgd && m->h.root == H.root &&
+                m->h.tab_rows == H.tab_rows && m->h.cond == H.cond;
+    for (int e = 0; same && e < H.n; ++e) {
+      const NodeDev &x = m->h.nodes[e], &y = H.nodes[e];
+      same = x.kind == y.kind && x.child0 == y.child0 && x.child1 == y.child1 && x.tab == y.tab && x.R == y.R &&
+             x.wgd == y.wgd && x.parent == y.parent && x.leafP == y.leafP && x.delta == y.delta;
+    }
+    if (!same) return fail(WHALE_EINVAL, "mixture components must share the batch's species-tree structure");
+  }
+  whale_ctx* c = m0->ctx;
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->stream;
+  {
+    const int rc = ensure_full(b);
+    if (rc != WHALE_OK) return rc;
+  }
+  const uint32_t nf = b->n_fam;
+  const bool fp32 = mode == WHALE_FP32_SCALED;
+  const bool grad = gradient != 0;
+  const AccLayout AL(H.tab_rows, H.n, H.n_wgd);
+  const int ng = 2 + 2 * H.n + H.n_wgd;  // a tail's output
+  // workspaces, sized as the batch's own
+  while ((int)b->mixw.size() < K) b->mixw.emplace_back(new whale_batch::MixWork());
+  std::vector<WorkRaw> Wj(K);
+  for (int j = 0; j < K; ++j) {
+    whale_batch::MixWork& x = *b->mixw[j];
+    HIPCHK(x.ell.alloc(b->ell.n));
+    HIPCHK(x.coef.alloc(b->coef.n));
+    HIPCHK(x.E.alloc(b->E.n));
+    HIPCHK(x.ibar.alloc(b->ibar.n));
+    HIPCHK(x.fam_ll.alloc(b->fam_ll.n));
+    HIPCHK(x.acc.alloc(b->acc.n));
+    HIPCHK(x.ticket.alloc(sizeof(uint32_t) * kTickets));
+    HIPCHK(x.flag.alloc(b->flag.n));
+    HIPCHK(x.fb.alloc(b->fb.n));
+    HIPCHK(x.seed.alloc(sizeof(double) * (nf + 1)));
+    HIPCHK(x.out.alloc(sizeof(double) * ng));
+    WorkRaw W;
+    W.ell = x.ell.p;
+    W.coef = x.coef.p;
+    W.E = x.E.as<int32_t>();
+    W.ibar = x.ibar.p;
+    W.fam_ll = x.fam_ll.as<double>();
+    W.acc = x.acc.as<double>();
+    W.ticket = x.ticket.as<uint32_t>();
+    W.flag = x.flag.as<uint32_t>();
+    W.fb = x.fb.as<uint32_t>();
+    W.gws = nullptr;
+    W.seed = x.seed.as<double>();  // written by the mixture kernel, read by the reverse sweep
+    Wj[j] = W;
+  }
+  // the component tables of every launch: [launch][component]
+  enum { kF = 0, kB = whale_batch::kClasses, kFbF = 2 * whale_batch::kClasses, kFbB, kRtF, kRtB, kSlots };
+  std::vector<MixComp> sel((size_t)kSlots * K);
+  for (int j = 0; j < K; ++j) {
+    const ModelDev M = comps[j]->dev();
+    const WorkRaw Wfb = b->fallback_of(Wj[j]);
+    for (int k = 0; k < whale_batch::kClasses; ++k) {
+      MixComp& f = sel[(size_t)(kF + k) * K + j];
+      f.M = M;
+      f.W = Wj[j];
+      f.W.ticket = Wj[j].ticket + k;
+      MixComp& r = sel[(size_t)(kB + k) * K + j];
+      r = f;
+      r.W.ticket = Wj[j].ticket + whale_batch::kClasses + k;
+    }
+    MixComp& a = sel[(size_t)kFbF * K + j];
+    a.M = M;
+    a.W = Wfb;
+    a.W.ticket = Wj[j].ticket + 2 * whale_batch::kClasses;
+    a.order = Wj[j].fb + 1;
+    a.count = Wj[j].fb;
+    MixComp& d = sel[(size_t)kFbB * K + j];
+    d = a;
+    d.W.ticket = Wj[j].ticket + 2 * whale_batch::kClasses + 1;
+    MixComp& e = sel[(size_t)kRtF * K + j];
+    e.M = M;
+    e.W = fp32 ? Wfb : Wj[j];
+    e.W.ticket = Wj[j].ticket + 2 * whale_batch::kClasses + 2;
+    MixComp& g = sel[(size_t)kRtB * K + j];
+    g = e;
+    g.W.ticket = Wj[j].ticket + 2 * whale_batch::kClasses + 3;
+  }
+  HIPCHK(b->mixsel.alloc(sizeof(MixComp) * sel.size()));
+  HIPCHK(hipMemcpy(b->mixsel.p, sel.data(), sizeof(MixComp) * sel.size(), hipMemcpyHostToDevice));
+  std::vector<double> lp(K);
+  for (int j = 0; j < K; ++j) lp[j] = std::log(prior[j]);
+  HIPCHK(b->mixprior.alloc(sizeof(double) * K));
+  HIPCHK(hipMemcpy(b->mixprior.p, lp.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+  HIPCHK(b->mixout.alloc(sizeof(double)));
+  HIPCHK(b->mixresp.alloc(sizeof(double) * (size_t)nf * K));
+  const MixComp* X = b->mixsel.as<MixComp>();
+  auto at = [&](int slot) { return X + (size_t)slot * K; };
+  const BatchDev B = b->dev();
+  const ModelDev M0 = m0->dev();
+  const bool store_all = grad;
+  for (int j = 0; j < K; ++j) {
+    HIPCHK(launch_fill_u32(st, Wj[j].ticket, 0u, kTickets));
+    if (grad) HIPCHK(launch_fill_f64(st, Wj[j].acc, 0.0, AL.n));
+    if (fp32) {
+      HIPCHK(launch_fill_u32(st, Wj[j].flag, 0u, (int)nf));
+      HIPCHK(launch_fill_u32(st, Wj[j].fb, 0u, 1));
+    }
+  }
+  // one direction: every size class as one launch over all components on the
+  // three streams, the families routed to FP64 as one launch beside them, and
+  // the FP64 pass over the families each component's FP32 pass flagged
+  const int* kOrder = kLayoutOrder[class_layout(b)];
+  const int* kLane = kLayoutLane[class_layout(b)];
+  auto direction = [&](int dir) -> int {
+    constexpr int kLanes = 3;
+    hipStream_t lanes[kLanes] = {st, c->side, c->side3};
+    bool used[kLanes] = {};
+    for (int k = 0; k < whale_batch::kClasses; ++k)
+      if (b->cls_hi[k] > b->cls_lo[k]) used[kLane[k]] = true;
+    const bool routed = b->route_lo < nf;
+    HIPCHK(hipEventRecord(c->fork[3 * dir], st));
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) HIPCHK(hipStreamWaitEvent(lanes[l], c->fork[3 * dir], 0));
+    if (routed) HIPCHK(hipStreamWaitEvent(c->side2, c->fork[3 * dir], 0));
+    for (int q = 0; q < whale_batch::kClasses; ++q) {
+      const int k = kOrder[q];
+      if (b->cls_hi[k] == b->cls_lo[k]) continue;
+      hipStream_t sk = lanes[kLane[k]];
+      if (dir == 0)
+        HIPCHK(launch_forward(mode, store_all, b->cfg[mode][0][k], sk, B, M0, Wj[0], -1, b->cls_lo[k], b->cls_hi[k],
+                              nullptr, at(kF + k), K));
+      else
+        HIPCHK(launch_backward(mode, b->cfg[mode][1][k], sk, B, M0, Wj[0], -1, b->cls_lo[k], b->cls_hi[k], nullptr,
+                               at(kB + k), K));
+    }
+    if (routed) {
+      const int rmode = fp32 ? WHALE_FP64_SCALED : mode;
+      if (dir == 0)
+        HIPCHK(launch_forward(rmode, store_all, b->cfg_route[mode][0], c->side2, B, M0, Wj[0], -1, b->route_lo, nf,
+                              nullptr, at(kRtF), K));
+      else
+        HIPCHK(launch_backward(rmode, b->cfg_route[mode][1], c->side2, B, M0, Wj[0], -1, b->route_lo, nf, nullptr,
+                               at(kRtB), K));
+      HIPCHK(hipEventRecord(c->fork[7], c->side2));
+      HIPCHK(hipStreamWaitEvent(st, c->fork[7], 0));
+    }
+    for (int l = 1; l < kLanes; ++l)
+      if (used[l]) {
+        HIPCHK(hipEventRecord(c->fork[3 * dir + l], lanes[l]));
+        HIPCHK(hipStreamWaitEvent(st, c->fork[3 * dir + l], 0));
+      }
+    if (fp32) {
+      BatchDev Bfb = B;
+      if (dir == 0)
+        HIPCHK(launch_forward(WHALE_FP64_SCALED, store_all, b->cfg_fb[0], st, Bfb, M0, Wj[0], -1, 0, 0, nullptr,
+                              at(kFbF), K));
+      else
+        HIPCHK(launch_backward(WHALE_FP64_SCALED, b->cfg_fb[1], st, Bfb, M0, Wj[0], -1, 0, 0, nullptr, at(kFbB), K));
+    }
+    return WHALE_OK;
+  };
+  int rc = direction(0);
+  if (rc != WHALE_OK) return rc;
+  // logsumexp over the components, the responsibilities as the reverse sweep's weights
+  HIPCHK(launch_mix(st, at(kF), K, b->mixprior.as<double>(), nf, H.cond, b->mixout.as<double>(),
+                    b->mixresp.as<double>()));
+  if (grad) {
+    if ((rc = direction(1)) != WHALE_OK) return rc;
+    for (int j = 0; j < K; ++j) {
+      const whale_model* m = comps[j];
+      HIPCHK(launch_tail(st, m->dev(), Wj[j].fam_ll, nf, Wj[j].acc, m->ebar.as<double>(), m->drow.as<double>(),
+                         m->dnode.as<double>(), m->dlv.as<int32_t>(), m->dnd.as<int32_t>(), m->n_dlevels, H.cond, 1,
+                         b->mixw[j]->out.as<double>(), Wj[j].seed + nf));
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(out_total, b->mixout.p, sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> col(nf + 1);
+  for (int j = 0; j < K; ++j) {
+    if (out_fam) {
+      HIPCHK(hipMemcpy(col.data(), Wj[j].fam_ll, sizeof(double) * nf, hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < nf; ++i) out_fam[(size_t)i * K + j] = col[i];
+    }
+    if (grad) {
+      std::vector<double> o(ng);
+      HIPCHK(hipMemcpy(o.data(), b->mixw[j]->out.p, sizeof(double) * ng, hipMemcpyDeviceToHost));
+      std::copy(o.begin() + 1, o.begin() + 1 + (ng - 1), out_grad + (size_t)j * (ng - 1));
+    }
+    if (out_dprior && grad) {
+      double sw = 0;
+      HIPCHK(hipMemcpy(&sw, Wj[j].seed + nf, sizeof(double), hipMemcpyDeviceToHost));
+      out_dprior[j] = sw / prior[j];
+    }
+  }
+  if (out_resp) HIPCHK(hipMemcpy(out_resp, b->mixresp.p, sizeof(double) * (size_t)nf * K, hipMemcpyDeviceToHost));
+  if (out_dprior && !grad) {
+    std::vector<double> r((size_t)nf * K);
+    HIPCHK(hipMemcpy(r.data(), b->mixresp.p, sizeof(double) * r.size(), hipMemcpyDeviceToHost));
+    for (int j = 0; j < K; ++j) {
+      double sw = 0;
+      for (uint32_t i = 0; i < nf; ++i) sw += r[(size_t)i * K + j];
+      out_dprior[j] = sw / prior[j];
+    }
   }
   return WHALE_OK;
 }

@@ -159,12 +159,32 @@ size_t bwd_shared_bytes(int mode, bool tabl, int tab_rows, int acc_n, int n, int
 // 1-block grid), otherwise waves take families order[fam_begin .. fam_end)
 // from the ticket.  fam_count, if not null, replaces fam_end with a count read
 // on the device (the FP64 pass over families listed by the FP32 pass).
+// One component of a mixture launch: with K components the grid is K times
+// the launch's, and workgroup w works for component w % K with that
+// component's model (tables), workspace (tickets included), and, when not
+// null, its own family order and count (the FP64 pass over families its
+// FP32 pass flagged).
+struct MixComp {
+  ModelDev M;
+  WorkRaw W;
+  const uint32_t* order = nullptr;
+  const uint32_t* count = nullptr;
+};
+constexpr int kMaxMix = 8;
+
 hipError_t launch_forward(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
                           const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
-                          uint32_t fam_end, const uint32_t* fam_count = nullptr);
+                          uint32_t fam_end, const uint32_t* fam_count = nullptr, const MixComp* X = nullptr,
+                          int K = 0);
 hipError_t launch_backward(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
                            const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
-                           const uint32_t* fam_count = nullptr);
+                           const uint32_t* fam_count = nullptr, const MixComp* X = nullptr, int K = 0);
+// per family i: ℓ_ij = fam_ll_j[i] + log prior_j − c_j over the K components
+// (c_j the component's conditioning term), r_ij = exp(ℓ_ij − logsumexp_j ℓ_ij)
+// into seed_j[i] and Σ_i r_ij into seed_j[n_fam]; out[0] = Σ_i logsumexp_j ℓ_ij
+// (−inf if not finite); resp (n_fam × K, may be null) = r
+hipError_t launch_mix(hipStream_t st, const MixComp* X, int K, const double* log_prior, uint32_t n_fam, int cond,
+                      double* out, double* resp);
 // per-wave slots of the fused forward + reverse launch: each holds one
 // family's stored rows (from byte 0), ī (from ibar_off) and record
 // coefficients (from coef_off)

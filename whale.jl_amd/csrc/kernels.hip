@@ -69,7 +69,7 @@ struct WorkDev {
   unsigned char* gws;
   double* acc_fam;    // deterministic gradients: per-family accumulator slots (null otherwise)
   const double* seed; // per-family weights of the reverse sweep (null: 1)
-  explicit WorkDev(const WorkRaw& w)
+  __host__ __device__ explicit WorkDev(const WorkRaw& w)
       : ell((Real*)w.ell), coef((Real*)w.coef), E(w.E), ibar((Real*)w.ibar), fam_ll(w.fam_ll), acc(w.acc),
         ticket(w.ticket), flag(w.flag), fb(w.fb), gws(w.gws), acc_fam(w.acc_fam), seed(w.seed) {}
 };
@@ -631,6 +631,17 @@ constexpr int kMaxFPW = 8;  // families (waves) per workgroup, at most
 // a wave mostly waits on its own LDS round trips, so a third wave per SIMD
 // hides more of them than the registers it costs
 constexpr int kWavesPerEU = 3;
+
+// a mixture launch's workgroup takes its component's model, workspace and,
+// when given, family order and count
+template <typename Real>
+__device__ inline void mix_select(const MixComp& x, BatchDev& B, ModelDev& M, WorkDev<Real>& W,
+                                  const uint32_t*& fam_count) {
+  M = x.M;
+  W = WorkDev<Real>(x.W);
+  if (x.order) B.order = x.order;
+  if (x.count) fam_count = x.count;
+}
 
 // The forward DP of family f by one wave: its stages (row buffers in the
 // wave's region wb), then its root.  ℓ is written through `ell` (the
@@ -1253,7 +1264,8 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
 template <typename Real, bool Scaled, bool StoreAll, bool TabL, bool Gws>
 __global__ __launch_bounds__(kWG* kMaxFPW, kWavesPerEU) void k_forward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                        int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
-                                                       const uint32_t* fam_count) {
+                                                       const uint32_t* fam_count, const MixComp* X, int K) {
+  if (K > 0) mix_select(X[blockIdx.x % (uint32_t)K], B, M, W, fam_count);
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x % kWG;
@@ -1870,8 +1882,9 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
 template <typename Real, bool Scaled, bool TabL, bool Gws>
 __global__ __launch_bounds__(kWG* kMaxFPW, kWavesPerEU) void k_backward(BatchDev B, ModelDev M, WorkDev<Real> W, LdsCfg L,
                                                         int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
-                                                        const uint32_t* fam_count) {
+                                                        const uint32_t* fam_count, const MixComp* X, int K) {
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
+  if (K > 0) mix_select(X[blockIdx.x % (uint32_t)K], B, M, W, fam_count);
   if (fam_count) fam_end = fam_begin + *fam_count;
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = threadIdx.x % kWG;
@@ -2159,6 +2172,87 @@ __global__ __launch_bounds__(kWG* kPoolWaves, kWavesPerEU) void k_pool(BatchDev 
 }
 
 // ---------------------------------------------------------------------------
+// The conditioning term c = log P(condition) (src/condition.jl:11-29) of the
+// current parameters and its derivatives in the root's ε, its children's ε
+// and η: RootCondition P = 1 − g(ε_f) − g(ε_g) + g(ε_ρ), NonExtinct
+// P = 1 − g(ε_ρ), g the root's geometric pgf; none: c = 0.
+struct CondTerms {
+  double c, dc_er, dc_ef, dc_eg, dc_eta;
+};
+__device__ inline CondTerms cond_terms(const ModelDev& M, int cond) {
+  const NodeDev& rn = M.node[M.root];
+  const double eta = M.par[3 * M.n + M.n_wgd];
+  const double er = eps_top(M, M.root);
+  const double ef = eps_top(M, rn.child0), eg = eps_top(M, rn.child1);
+  auto gpgf = [&](double x) { return eta * x / (1.0 - (1.0 - eta) * x); };
+  auto dg_ds = [&](double x) { double D = 1.0 - (1.0 - eta) * x; return eta / (D * D); };
+  auto dg_deta = [&](double x) { double D = 1.0 - (1.0 - eta) * x; return x * (1.0 - x) / (D * D); };
+  CondTerms T{0, 0, 0, 0, 0};
+  if (cond == WHALE_COND_ROOT) {
+    const double P = 1.0 - gpgf(ef) - gpgf(eg) + gpgf(er);
+    T.c = P > 0 ? log(P) : -INFINITY;
+    T.dc_ef = -dg_ds(ef) / P;
+    T.dc_eg = -dg_ds(eg) / P;
+    T.dc_er = dg_ds(er) / P;
+    T.dc_eta = (-dg_deta(ef) - dg_deta(eg) + dg_deta(er)) / P;
+  } else if (cond == WHALE_COND_NONEXTINCT) {
+    const double P = 1.0 - gpgf(er);
+    T.c = log(P);
+    T.dc_er = -dg_ds(er) / P;
+    T.dc_eta = -dg_deta(er) / P;
+  }
+  return T;
+}
+
+// k_mix: one workgroup.  The mixture's per-family log-sum-exp over its
+// components (src/core.jl:66-76) and the responsibilities that weight each
+// component's reverse sweep.
+__global__ __launch_bounds__(1024) void k_mix(const MixComp* X, int K, const double* log_prior, uint32_t n_fam,
+                                              int cond, double* out, double* resp) {
+  __shared__ double red[1024];
+  __shared__ double s_c[kMaxMix];
+  __shared__ double s_w[kMaxMix][32];
+  const int t = threadIdx.x;
+  if (t < K) s_c[t] = cond_terms(X[t].M, cond).c;
+  for (int i = t; i < kMaxMix * 32; i += blockDim.x) s_w[i / 32][i % 32] = 0.0;
+  __syncthreads();
+  double tot = 0.0, wsum[kMaxMix];
+  for (int j = 0; j < kMaxMix; ++j) wsum[j] = 0.0;
+  for (uint32_t i = t; i < n_fam; i += blockDim.x) {
+    double l[kMaxMix], mx = -INFINITY;
+    for (int j = 0; j < K; ++j) {
+      l[j] = X[j].W.fam_ll[i] + log_prior[j] - s_c[j];
+      mx = fmax(mx, l[j]);
+    }
+    double lse = -INFINITY;
+    if (isfinite(mx)) {
+      double acc = 0.0;
+      for (int j = 0; j < K; ++j) acc += exp(l[j] - mx);
+      lse = mx + log(acc);
+    }
+    for (int j = 0; j < K; ++j) {
+      const double r = isfinite(lse) ? exp(l[j] - lse) : 0.0;
+      const_cast<double*>(X[j].W.seed)[i] = r;
+      wsum[j] += r;
+      if (resp) resp[(size_t)i * K + j] = r;
+    }
+    tot += lse;
+  }
+  red[t] = tot;
+  for (int j = 0; j < K; ++j) atomicAdd(&s_w[j][t % 32], wsum[j]);
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t < K) {
+    double sw = 0.0;
+    for (int q = 0; q < 32; ++q) sw += s_w[t][q];
+    const_cast<double*>(X[t].W.seed)[n_fam] = sw;
+  }
+  if (t == 0) out[0] = isfinite(red[0]) ? red[0] : -INFINITY;
+}
+
 constexpr int kTailThreads = 1024;
 
 // k_tail: one workgroup.  Σℓ over the families, the conditioning term, and
@@ -2214,24 +2308,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(ModelDev M, const double*
   const NodeDev& rn = M.node[M.root];
   const double eta = M.par[3 * M.n + M.n_wgd];
   const double er = eps_top(M, M.root);
-  const double ef = eps_top(M, rn.child0), eg = eps_top(M, rn.child1);
-  auto gpgf = [&](double x) { return eta * x / (1.0 - (1.0 - eta) * x); };
-  auto dg_ds = [&](double x) { double D = 1.0 - (1.0 - eta) * x; return eta / (D * D); };
-  auto dg_deta = [&](double x) { double D = 1.0 - (1.0 - eta) * x; return x * (1.0 - x) / (D * D); };
-  double c = 0, dc_er = 0, dc_ef = 0, dc_eg = 0, dc_eta = 0;
-  if (cond == WHALE_COND_ROOT) {
-    double P = 1.0 - gpgf(ef) - gpgf(eg) + gpgf(er);
-    c = P > 0 ? log(P) : -INFINITY;
-    dc_ef = -dg_ds(ef) / P;
-    dc_eg = -dg_ds(eg) / P;
-    dc_er = dg_ds(er) / P;
-    dc_eta = (-dg_deta(ef) - dg_deta(eg) + dg_deta(er)) / P;
-  } else if (cond == WHALE_COND_NONEXTINCT) {
-    double P = 1.0 - gpgf(er);
-    c = log(P);
-    dc_er = -dg_ds(er) / P;
-    dc_eta = -dg_deta(er) / P;
-  }
+  const CondTerms CT = cond_terms(M, cond);
+  const double c = CT.c, dc_er = CT.dc_er, dc_ef = CT.dc_ef, dc_eg = CT.dc_eg, dc_eta = CT.dc_eta;
   const double N = (double)n_fam;
   if (t == 0) {
     const double tot = red[0] - N * c;
@@ -2330,63 +2408,69 @@ static hipError_t go(K kern, int grid, int fpw, size_t lds, hipStream_t st, Args
 template <typename R, bool S, bool A>
 static hipError_t fwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
                               const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
-                              const uint32_t* fam_count) {
+                              const uint32_t* fam_count, const MixComp* X, int K) {
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
-  if (C.fpw < 1 || C.fpw > kMaxFPW) return hipErrorInvalidValue;
+  if (C.fpw < 1 || C.fpw > kMaxFPW || K < 0 || K > kMaxMix || (K > 0 && (!X || fam_single >= 0)))
+    return hipErrorInvalidValue;
+  const int grid = C.grid * (K > 0 ? K : 1);
   if (C.gws) {  // stage buffers in the global workspace: one wave per workgroup, tables from global memory
-    if (C.fpw != 1 || C.tabl || !W.gws) return hipErrorInvalidValue;
-    return go(k_forward<R, S, A, false, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
-              fam_count);
+    if (C.fpw != 1 || C.tabl || !W.gws || K > 0) return hipErrorInvalidValue;
+    return go(k_forward<R, S, A, false, true>, grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count, X, K);
   }
   if (C.tabl)
-    return go(k_forward<R, S, A, true, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
-              fam_count);
-  return go(k_forward<R, S, A, false, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
-            fam_count);
+    return go(k_forward<R, S, A, true, false>, grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count, X, K);
+  return go(k_forward<R, S, A, false, false>, grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+            fam_count, X, K);
 }
 
 template <typename R, bool S>
 static hipError_t bwd_variant(const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
                               const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
-                              const uint32_t* fam_count) {
+                              const uint32_t* fam_count, const MixComp* X, int K) {
   const size_t lds = C.lds();
   const WorkDev<R> w(W);
-  if (C.fpw < 1 || C.fpw > kMaxFPW) return hipErrorInvalidValue;
+  if (C.fpw < 1 || C.fpw > kMaxFPW || K < 0 || K > kMaxMix || (K > 0 && (!X || fam_single >= 0)))
+    return hipErrorInvalidValue;
+  const int grid = C.grid * (K > 0 ? K : 1);
   if (C.gws) {
-    if (C.fpw != 1 || !W.gws) return hipErrorInvalidValue;
+    if (C.fpw != 1 || !W.gws || K > 0) return hipErrorInvalidValue;
     if (C.tabl)
-      return go(k_backward<R, S, true, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
-                fam_count);
-    return go(k_backward<R, S, false, true>, C.grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
-              fam_count);
+      return go(k_backward<R, S, true, true>, grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+                fam_count, X, K);
+    return go(k_backward<R, S, false, true>, grid, 1, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count, X, K);
   }
   if (C.tabl)
-    return go(k_backward<R, S, true, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
-              fam_count);
-  return go(k_backward<R, S, false, false>, C.grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
-            fam_count);
+    return go(k_backward<R, S, true, false>, grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+              fam_count, X, K);
+  return go(k_backward<R, S, false, false>, grid, C.fpw, lds, st, B, M, w, C.L, fam_single, fam_begin, fam_end,
+            fam_count, X, K);
 }
 
 hipError_t launch_forward(int mode, bool store_all, const LaunchCfg& C, hipStream_t st, const BatchDev& B,
                           const ModelDev& M, const WorkRaw& W, int32_t fam_single, uint32_t fam_begin,
-                          uint32_t fam_end, const uint32_t* fam_count) {
+                          uint32_t fam_end, const uint32_t* fam_count, const MixComp* X, int K) {
   if (mode == WHALE_FP64_PLAIN)
-    return store_all ? fwd_variant<double, false, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count)
-                     : fwd_variant<double, false, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+    return store_all ? fwd_variant<double, false, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K)
+                     : fwd_variant<double, false, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K);
   if (mode == WHALE_FP32_SCALED)
-    return store_all ? fwd_variant<float, true, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count)
-                     : fwd_variant<float, true, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
-  return store_all ? fwd_variant<double, true, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count)
-                   : fwd_variant<double, true, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+    return store_all ? fwd_variant<float, true, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K)
+                     : fwd_variant<float, true, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K);
+  return store_all ? fwd_variant<double, true, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K)
+                   : fwd_variant<double, true, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K);
 }
 
 hipError_t launch_backward(int mode, const LaunchCfg& C, hipStream_t st, const BatchDev& B, const ModelDev& M,
                            const WorkRaw& W, int32_t fam_single, uint32_t fam_begin, uint32_t fam_end,
-                           const uint32_t* fam_count) {
-  if (mode == WHALE_FP64_PLAIN) return bwd_variant<double, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
-  if (mode == WHALE_FP32_SCALED) return bwd_variant<float, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
-  return bwd_variant<double, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count);
+                           const uint32_t* fam_count, const MixComp* X, int K) {
+  if (mode == WHALE_FP64_PLAIN)
+    return bwd_variant<double, false>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K);
+  if (mode == WHALE_FP32_SCALED)
+    return bwd_variant<float, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K);
+  return bwd_variant<double, true>(C, st, B, M, W, fam_single, fam_begin, fam_end, fam_count, X, K);
 }
 
 template <typename R, bool S>
@@ -2433,6 +2517,13 @@ hipError_t launch_pool(int mode, bool store_all, const LaunchCfg& C, hipStream_t
   if (mode == WHALE_FP64_PLAIN) return pool_variant<double, false>(store_all, C, st, B, M, W, fam_begin, fam_end, grad);
   if (mode == WHALE_FP32_SCALED) return pool_variant<float, true>(store_all, C, st, B, M, W, fam_begin, fam_end, grad);
   return pool_variant<double, true>(store_all, C, st, B, M, W, fam_begin, fam_end, grad);
+}
+
+hipError_t launch_mix(hipStream_t st, const MixComp* X, int K, const double* log_prior, uint32_t n_fam, int cond,
+                      double* out, double* resp) {
+  if (K < 1 || K > kMaxMix) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mix, dim3(1), dim3(1024), 0, st, X, K, log_prior, n_fam, cond, out, resp);
+  return hipGetLastError();
 }
 
 hipError_t launch_tables(hipStream_t st, const ModelDev& M, double* eps, double* phi, double* psi, double* drow,

@@ -5,19 +5,27 @@ the sampling condition of a model (src/condition.jl:11-29).
     MixtureModel(components, prior)     logpdf: ℓhood(Σ_i logsumexp_j (ℓ_ij + log p_j − c_j))
     ModelArray(models)                  logpdf: Σ_i logpdf(models[i], ccds[i])
 
-Each component is evaluated over the whole batch by the batched device
-launches (the components share the species tree, so they share the
-device-resident batch); the per-family values are combined on the host.
+When the components share one species-tree structure and a device-side
+condition, the mixture is evaluated by whale_mixture_eval: every size-class
+launch works for all components at once (a workgroup per component), the
+log-sum-exp and responsibilities are formed on the device, and the gradient's
+reverse sweeps of all components run as one launch per class with the
+responsibilities as the families' weights.  Otherwise each component is
+evaluated by the batched launches and the per-family values are combined on
+the host.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from typing import Sequence
 
 import numpy as np
 
-from .ccd import CCD
-from .core import gradient, logpdf, logpdf_families
+from . import _lib
+from ._lib import check, dblp, ptr
+from .ccd import CCD, device_batch
+from .core import _mode, gradient, logpdf, logpdf_families
 from .model import NoCondition, NonExtinctCondition, NowhereExtinctCondition, RootCondition
 
 
@@ -154,13 +162,77 @@ class MixtureModel:
             cols.append(fam + math.log(pj) - condition(wm))
         return np.stack(cols, axis=1)
 
+    def on_device(self, ccds=None) -> bool:
+        """Whether whale_mixture_eval evaluates this mixture: the components
+        share one structure and a device-side condition, at most 8 of them, and
+        (given `ccds`) the batch has no family in global memory."""
+        s = self.components[0].s
+        ok = (len(self.components) <= 8 and all(wm.s is s for wm in self.components)
+              and not getattr(s.condition, "host", False))
+        if ok and ccds is not None and not isinstance(ccds, CCD):
+            ok = device_batch(self.components[0], ccds).hbm_families() == 0
+        return ok
+
+    def _device_eval(self, ccds, mode: str, grad: bool):
+        """whale_mixture_eval: (total, ℓ (n × K), r (n × K), per-component
+        natural-scale gradients or None, d/dprior)."""
+        comps = self.components
+        K = len(comps)
+        s = comps[0].s
+        b = device_batch(comps[0], ccds)
+        hs = s.mix_handles(K)
+        for j, wm in enumerate(comps):
+            hs[j][1] = wm._set_params(hs[j][0], hs[j][1])
+        arr = (C.c_void_p * K)(*[h.value for h, _ in hs])
+        prior = np.ascontiguousarray(self.prior, dtype=np.float64)
+        n, nn, nw = b.n, s.n, s.n_wgd
+        fam = np.zeros(n * K)
+        resp = np.zeros(n * K)
+        g = np.zeros(K * (2 * nn + nw + 1))
+        dprior = np.zeros(K)
+        tot = C.c_double()
+        check(_lib.lib().whale_mixture_eval(b.h, arr, K, ptr(prior, dblp), _mode(mode), int(grad), C.byref(tot),
+                                            ptr(fam, dblp), ptr(resp, dblp), ptr(g, dblp) if grad else None,
+                                            ptr(dprior, dblp)), "whale_mixture_eval")
+        return tot.value, fam.reshape(n, K), resp.reshape(n, K), (g.reshape(K, -1) if grad else None), dprior
+
     def logpdf(self, ccds, mode: str = "fp64") -> float:
-        tot = float(np.sum(_logsumexp_rows(self.family_matrix(ccds, mode))))
+        if self.on_device(ccds):
+            tot = self._device_eval(ccds, mode, False)[0]
+        else:
+            tot = float(np.sum(_logsumexp_rows(self.family_matrix(ccds, mode))))
         return tot if math.isfinite(tot) else -math.inf
 
     def responsibilities(self, ccds, mode: str = "fp64"):
         """(logpdf, r): r[i, j] the posterior probability that family i comes
         from component j (rows of families no component can produce are 0)."""
+        if self.on_device(ccds):
+            tot, _, R, _, _ = self._device_eval(ccds, mode, False)
+            return (tot if math.isfinite(tot) else -math.inf), R
+        return self._host_resp(ccds, mode)
+
+    def gradient(self, ccds, mode: str = "fp64", device: bool = True):
+        """(value, [gradient of each component], d/dprior): the derivative of
+        logpdf(MixtureModel, ccds) (src/core.jl:66-76) with respect to each
+        component's rates vector (as `gradient` lays it out) and to the prior
+        weights, i.e. Σ_i r_ij ∂(ℓ_ij − c_j) and Σ_i r_ij / prior[j].  On the
+        device path all components run in the same launches; otherwise (or with
+        device=False) one evaluation per component gives ℓ_ij and one reverse
+        sweep per component, weighted by the responsibilities, the rest."""
+        if device and self.on_device(ccds):
+            tot, _, R, g, dprior = self._device_eval(ccds, mode, True)
+            nn, nw = self.components[0].s.n, self.components[0].s.n_wgd
+            grads = []
+            for j, wm in enumerate(self.components):
+                row = g[j]
+                dq = row[2 * nn:2 * nn + nw].copy() if nw else np.zeros(1)
+                grads.append(wm.chain_gradient(row[:nn].copy(), row[nn:2 * nn].copy(), dq, row[-1]))
+            return (tot if math.isfinite(tot) else -math.inf), grads, dprior
+        tot, R = self._host_resp(ccds, mode)
+        grads = [gradient(wm, ccds, mode=mode, weights=R[:, j])[1] for j, wm in enumerate(self.components)]
+        return tot, grads, R.sum(axis=0) / self.prior
+
+    def _host_resp(self, ccds, mode):
         F = self.family_matrix(ccds, mode)
         lse = _logsumexp_rows(F)
         R = np.zeros_like(F)
@@ -168,18 +240,6 @@ class MixtureModel:
         R[ok] = np.exp(F[ok] - lse[ok, None])
         tot = float(np.sum(lse))
         return (tot if math.isfinite(tot) else -math.inf), R
-
-    def gradient(self, ccds, mode: str = "fp64"):
-        """(value, [gradient of each component], d/dprior): the derivative of
-        logpdf(MixtureModel, ccds) (src/core.jl:66-76) with respect to each
-        component's rates vector (as `gradient` lays it out) and to the prior
-        weights, i.e. Σ_i r_ij ∂(ℓ_ij − c_j) and Σ_i r_ij / prior[j].  One
-        device evaluation per component gives ℓ_ij, one reverse sweep per
-        component with the responsibilities r_ij as the families' weights
-        gives the rest."""
-        tot, R = self.responsibilities(ccds, mode)
-        grads = [gradient(wm, ccds, mode=mode, weights=R[:, j])[1] for j, wm in enumerate(self.components)]
-        return tot, grads, R.sum(axis=0) / self.prior
 
 
 class ModelArray:

@@ -201,41 +201,58 @@ class _Structure:
 
     def handle(self):
         if self._h is None:
-            n = self.n
-            par = np.array([d.parent - 1 for d in self.nodes], np.int32)
-            kind = np.zeros(n, np.int32)
-            c0 = np.full(n, -1, np.int32)
-            c1 = np.full(n, -1, np.int32)
-            for d in self.nodes:
-                if d.isroot():
-                    kind[d.id - 1] = _lib.NODE_ROOT
-                elif d.wgd:
-                    kind[d.id - 1] = _lib.NODE_WGD
-                elif d.isleaf():
-                    kind[d.id - 1] = _lib.NODE_LEAF
-                else:
-                    kind[d.id - 1] = _lib.NODE_SPECIATION
-                if d.children:
-                    c0[d.id - 1] = d.children[0] - 1
-                if len(d.children) > 1:
-                    c1[d.id - 1] = d.children[1] - 1
-            ks = np.array([d.k for d in self.nodes], np.int32)
-            dl = np.array([d.delta for d in self.nodes], float)
-            lp = np.array([d.leafP for d in self.nodes], float)
-            wi = np.array([d.wgdid - 1 if d.wgd else -1 for d in self.nodes], np.int32)
-            ls = np.array([(min(d.clade) - 1) if d.isleaf() else -1 for d in self.nodes], np.int32)
-            order = np.array([i - 1 for i in self.order], np.int32)
-            h = C.c_void_p()
-            check(_lib.lib().whale_model_create(context(), n, ptr(par, i32p), ptr(kind, i32p), ptr(c0, i32p),
-                                                ptr(c1, i32p), ptr(ks, i32p), ptr(dl, dblp), ptr(lp, dblp),
-                                                ptr(wi, i32p), ptr(ls, i32p), n, ptr(order, i32p),
-                                                self.condition.code, C.byref(h)), "whale_model_create")
-            self._h = h
+            self._h = self._create()
             self._current = None
         return self._h
 
+    def mix_handles(self, k: int):
+        """k further device models of this structure, each with its own
+        parameters (the components of a device mixture evaluation): a list of
+        [handle, key of the parameters it holds]."""
+        mix = self.__dict__.setdefault("_mix", [])
+        while len(mix) < k:
+            mix.append([self._create(), None])
+        return mix[:k]
+
+    def _create(self):
+        n = self.n
+        par = np.array([d.parent - 1 for d in self.nodes], np.int32)
+        kind = np.zeros(n, np.int32)
+        c0 = np.full(n, -1, np.int32)
+        c1 = np.full(n, -1, np.int32)
+        for d in self.nodes:
+            if d.isroot():
+                kind[d.id - 1] = _lib.NODE_ROOT
+            elif d.wgd:
+                kind[d.id - 1] = _lib.NODE_WGD
+            elif d.isleaf():
+                kind[d.id - 1] = _lib.NODE_LEAF
+            else:
+                kind[d.id - 1] = _lib.NODE_SPECIATION
+            if d.children:
+                c0[d.id - 1] = d.children[0] - 1
+            if len(d.children) > 1:
+                c1[d.id - 1] = d.children[1] - 1
+        ks = np.array([d.k for d in self.nodes], np.int32)
+        dl = np.array([d.delta for d in self.nodes], float)
+        lp = np.array([d.leafP for d in self.nodes], float)
+        wi = np.array([d.wgdid - 1 if d.wgd else -1 for d in self.nodes], np.int32)
+        ls = np.array([(min(d.clade) - 1) if d.isleaf() else -1 for d in self.nodes], np.int32)
+        order = np.array([i - 1 for i in self.order], np.int32)
+        h = C.c_void_p()
+        check(_lib.lib().whale_model_create(context(), n, ptr(par, i32p), ptr(kind, i32p), ptr(c0, i32p),
+                                            ptr(c1, i32p), ptr(ks, i32p), ptr(dl, dblp), ptr(lp, dblp),
+                                            ptr(wi, i32p), ptr(ls, i32p), n, ptr(order, i32p),
+                                            self.condition.code, C.byref(h)), "whale_model_create")
+        return h
+
     def __del__(self):
-        if getattr(self, "_h", None) is not None and _lib._lib is not None:
+        if _lib._lib is None:
+            return
+        for h, _ in self.__dict__.get("_mix", []):
+            _lib.lib().whale_model_destroy(h)
+        self.__dict__["_mix"] = []
+        if getattr(self, "_h", None) is not None:
             _lib.lib().whale_model_destroy(self._h)
             self._h = None
 
@@ -306,14 +323,19 @@ class WhaleModel:
     def _sync(self):
         """Build this parameter point's slice tables on the device (once)."""
         h = self.s.handle()
+        self.s._current = self._set_params(h, self.s._current)
+        return h
+
+    def _set_params(self, h, current):
+        """This parameter point's tables on device model `h` unless `current`
+        (the key of what h holds) is already this point; returns the key."""
         lam, mu, q, eta, p, _ = self.node_params()
         q = np.ascontiguousarray(q if len(q) else np.zeros(1))
         key = (lam.tobytes(), mu.tobytes(), q.tobytes(), eta, p.tobytes())
-        if self.s._current != key:
+        if current != key:
             check(_lib.lib().whale_model_set_params(h, ptr(lam, dblp), ptr(mu, dblp), ptr(q, dblp), len(q), eta,
                                                     ptr(p, dblp)), "whale_model_set_params")
-            self.s._current = key
-        return h
+        return key
 
     def tables(self):
         """Device slice tables, per node id: (ε, φ, ψ) arrays of k+1 rows."""
