@@ -378,6 +378,10 @@ def main():
                    "families": n_fam, "cells": st["cells"], "split_terms": st["split_terms"]},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": hbm, "kernel": dom,
+                     "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, scaled by the "
+                                        "counters' reading of a 256 MiB copy, committed in profiles/"
+                                        "config4_fp32_gradient_hbm_traffic.md (a profiler run of its own, not "
+                                        "this process)") if hbm else None,
                      "algorithmic_bytes": traffic_bytes, "duration_ms": dom_ms,
                      "duration": "HIP events on the library's stream around the kernel's phase (all size-class "
                                  "launches and the FP64 families, several streams) of eager evaluations, median"},
