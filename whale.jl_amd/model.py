@@ -289,6 +289,16 @@ class WhaleModel:
     def condition(self):
         return self.s.condition
 
+    def setsamplingp(self, d: Dict[str, float]):
+        """setsamplingp!(model, dict) (src/model.jl:248-255): every leaf's
+        sampling probability p from `d` by leaf name (0 for a leaf not in
+        `d`), stored in the rates by leaf id."""
+        leaves = [x for x in self.s.nodes if x.isleaf()]
+        p = list(self.rates.p) if len(self.rates.p) else [0.0] * len(leaves)
+        for x in leaves:
+            p[x.id - 1] = float(d.get(x.name, 0.0))
+        self.rates.p = p
+
     # -- getθ (src/rmodels.jl:31-33, 55-64) on the natural scale, per node
     def node_params(self):
         s, r = self.s, self.rates
