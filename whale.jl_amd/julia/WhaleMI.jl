@@ -13,12 +13,14 @@
 #   WhaleMI.logpdf(g, wm)                    # logpdf(wm, ccd), src/core.jl:58-64
 #   v, ∇ = WhaleMI.gradient(g, wm)           # ∂/∂[λ; μ; q; η] (the rates' own vector)
 #   Zygote.gradient(θ -> WhaleMI.loglik(g, wm, θ), θ)   # through the rrule below
+#   WhaleMI.mixture(g, MixtureModel([wm(θ1), wm(θ2)], [0.7, 0.3]); gradient=true)
 #
 # This file is not run in this repository's tests (its image has no Julia);
 # the C ABI it binds is what the tests exercise through ctypes.
 module WhaleMI
 
 using Whale, NewickTree
+using Distributions: MixtureModel, probs
 using ChainRulesCore
 
 const libwhale = joinpath(@__DIR__, "..", "libwhale_amd.so")
@@ -48,6 +50,7 @@ mutable struct GPUWhale
     nnodes::Int
     nwgd::Int
     nfam::Int
+    mix::Vector{Ptr{Cvoid}}   # device models of mixture components (same structure)
 end
 
 condcode(c) = c isa Whale.RootCondition ? Int32(0) : c isa Whale.NonExtinctCondition ? Int32(1) : Int32(2)
@@ -101,15 +104,19 @@ function device(wm::WhaleModel, ccds::AbstractVector{<:CCD}; dev::Integer=0)
     ctx = Ctx(dev)
     m = device_model(ctx, wm)
     b = device_batch(m, ccds)
-    g = GPUWhale(ctx, m, b, length(wm.order), length(wm.rates.q), length(ccds))
+    g = GPUWhale(ctx, m, b, length(wm.order), length(wm.rates.q), length(ccds), Ptr{Cvoid}[])
     finalizer(g) do x
         ccall((:whale_batch_destroy, libwhale), Cint, (Ptr{Cvoid},), x.batch)
+        for h in x.mix
+            ccall((:whale_model_destroy, libwhale), Cint, (Ptr{Cvoid},), h)
+        end
         ccall((:whale_model_destroy, libwhale), Cint, (Ptr{Cvoid},), x.model)
     end
 end
 
 # (m::WhaleModel)(θ): getθ per node on the natural scale (src/rmodels.jl:31-64)
-function set_params!(g::GPUWhale, wm::WhaleModel)
+set_params!(g::GPUWhale, wm::WhaleModel) = set_params!(g.model, g, wm)
+function set_params!(h::Ptr{Cvoid}, g::GPUWhale, wm::WhaleModel)
     n = g.nnodes
     θ = [Whale.getθ(wm.rates, wm[i]) for i=1:n]
     λ = Float64[t.λ for t in θ]
@@ -118,7 +125,7 @@ function set_params!(g::GPUWhale, wm::WhaleModel)
     q = Float64.(wm.rates.q)
     check(ccall((:whale_model_set_params, libwhale), Cint,
         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int32, Float64, Ptr{Float64}),
-        g.model, λ, μ, isempty(q) ? [0.0] : q, length(q), Float64(wm.rates.η), p))
+        h, λ, μ, isempty(q) ? [0.0] : q, length(q), Float64(wm.rates.η), p))
 end
 
 """logpdf(wm, ccds) (src/core.jl:58-64): the conditioned batch log-likelihood,
@@ -179,6 +186,37 @@ function ChainRulesCore.rrule(::typeof(loglik), g::GPUWhale, wm::WhaleModel, θ:
     v, ∇ = gradient(g, wm(unpack(wm, θ)); mode=mode)
     pullback(ȳ) = (NoTangent(), NoTangent(), NoTangent(), ȳ .* ∇)
     v, pullback
+end
+
+"""mixture(g, mm; gradient): logpdf(mm::MixtureModel, ccds) (src/core.jl:66-76)
+for components sharing g's tree layout, every size-class launch working for
+all components (whale_mixture_eval).  Returns the value and the
+responsibilities (families × components); with `gradient`, also each
+component's gradient in its rates' vector and d/dprior."""
+function mixture(g::GPUWhale, mm::MixtureModel; mode=FP32_SCALED, gradient::Bool=false)
+    comps = mm.components
+    K = length(comps)
+    while length(g.mix) < K
+        push!(g.mix, device_model(g.ctx, comps[1]))
+    end
+    for j in 1:K
+        set_params!(g.mix[j], g, comps[j])
+    end
+    prior = Float64.(probs(mm))
+    n, nw = g.nnodes, g.nwgd
+    ng = 2n + nw + 1
+    tot = Ref(0.0)
+    resp = zeros(K, g.nfam)          # family-major in C: r[i, j] at j + K(i-1)
+    grad = zeros(ng, K)
+    dprior = zeros(K)
+    check(ccall((:whale_mixture_eval, libwhale), Cint,
+        (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Int32, Ptr{Float64}, Int32, Int32, Ref{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+        g.batch, g.mix, K, prior, Int32(mode), Int32(gradient), tot, C_NULL, resp,
+        gradient ? grad : C_NULL, dprior))
+    gradient || return tot[], permutedims(resp)
+    ∇ = [chain(comps[j], grad[1:n, j], grad[n+1:2n, j], grad[2n+1:2n+nw, j], grad[end, j]) for j in 1:K]
+    tot[], permutedims(resp), ∇, dprior
 end
 
 """backtrack(g, n; seed): n reconciled trees per family from the ℓ the last
