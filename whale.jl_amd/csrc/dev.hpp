@@ -57,7 +57,7 @@ struct WorkRaw {
   // per-family weights of the reverse sweep (null: 1): family f's adjoint is
   // seeded with seed[f] / L_f, and seed[n_fam] = Σ seed scales the
   // conditioning term's derivative (mixture responsibilities)
-  const double* seed = nullptr;
+  double* seed = nullptr;
 };
 
 // gradient accumulator layout (doubles)

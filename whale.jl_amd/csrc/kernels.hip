@@ -68,7 +68,7 @@ struct WorkDev {
   uint32_t* fb;
   unsigned char* gws;
   double* acc_fam;    // deterministic gradients: per-family accumulator slots (null otherwise)
-  const double* seed; // per-family weights of the reverse sweep (null: 1)
+  double* seed;       // per-family weights of the reverse sweep (null: 1)
   __host__ __device__ explicit WorkDev(const WorkRaw& w)
       : ell((Real*)w.ell), coef((Real*)w.coef), E(w.E), ibar((Real*)w.ibar), fam_ll(w.fam_ll), acc(w.acc),
         ticket(w.ticket), flag(w.flag), fb(w.fb), gws(w.gws), acc_fam(w.acc_fam), seed(w.seed) {}
@@ -2232,7 +2232,7 @@ __global__ __launch_bounds__(1024) void k_mix(const MixComp* X, int K, const dou
     }
     for (int j = 0; j < K; ++j) {
       const double r = isfinite(lse) ? exp(l[j] - lse) : 0.0;
-      const_cast<double*>(X[j].W.seed)[i] = r;
+      X[j].W.seed[i] = r;
       wsum[j] += r;
       if (resp) resp[(size_t)i * K + j] = r;
     }
@@ -2248,7 +2248,7 @@ __global__ __launch_bounds__(1024) void k_mix(const MixComp* X, int K, const dou
   if (t < K) {
     double sw = 0.0;
     for (int q = 0; q < 32; ++q) sw += s_w[t][q];
-    const_cast<double*>(X[t].W.seed)[n_fam] = sw;
+    X[t].W.seed[n_fam] = sw;
   }
   if (t == 0) out[0] = isfinite(red[0]) ? red[0] : -INFINITY;
 }
