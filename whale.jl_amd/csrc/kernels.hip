@@ -461,6 +461,21 @@ __device__ __forceinline__ SegInfo seg_info(const BatchDev& B, const ModelDev& M
   return t;
 }
 
+// A generic pointer to global memory as a global-address-space pointer: its
+// loads and stores are global ones, which LDS accesses need not wait for (a
+// generic access counts against both the vector-memory and the LDS
+// counters, so a later wait for LDS also waits for it).  Only for pointers
+// known to address global memory.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> glob(T* p) {
+  return (gptr<T>)p;
+}
+
+// items (records) per lane that a stage's set-up loads at once
+constexpr int kStageU = 4;
+
 struct SegTab {
   SegInfo mine;
   uint32_t seg0;
@@ -650,11 +665,13 @@ __device__ inline void mix_select(const MixComp& x, BatchDev& B, ModelDev& M, Wo
 template <typename Real, bool Scaled, bool StoreAll, bool TabL, bool Gws>
 __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M, const WorkDev<Real>& W,
                                            const LdsCfg& L, uint32_t f, unsigned char* const wb, Real* sTab,
-                                           const NodeC<Real>& NC, uint32_t lane, Real* ell, Real* coef,
+                                           const NodeC<Real>& NC, uint32_t lane, Real* ell_arg, Real* coef_arg,
                                            uint32_t coef_base) {
+  const gptr<Real> ell = glob(ell_arg);
+  const gptr<Real> coef = glob(coef_arg);
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   const FamDev F = B.fam[f];
-  int32_t* E = W.E + F.clade0;
+  const gptr<int32_t> E = glob(W.E + F.clade0);
   if (Scaled)
     for (uint32_t c = lane; c < F.n_clades; c += kWG) E[c] = kUndef;
   // ---------------- non-root stages
@@ -678,28 +695,70 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
     wave_sync_mem();  // ℓ and exponents of the stages below
     SegTab ST;
     ST.load(B, M, S.seg0, S.n_segs, lane);
-    for (uint32_t i = lane; i < n; i += kWG) {
-      const uint32_t ia = i0 + i;
-      const int32_t cf = B.it_cf[ia], cg = B.it_cg[ia];
-      sSegL[i] = (uint16_t)(B.it_seg[ia] - S.seg0);
-      bF[i] = cf >= 0 ? ell[cf] : Real(0);
-      bB[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops (bB is free until row 1)
-      sAcc[i] = 0;
-      sRO[i] = B.it_rec[ia] - rb;
-      if (Scaled) {
-        const int e = E[B.it_gid[ia]];
-        sEi[i] = e;
-        sFlag[i] = e == kUndef ? (kPending | kYoung) : 0;
+    // kStageU items per lane at once: all their index loads, then all the
+    // loads of ℓ and exponents they name (in-bounds dummies past n), then
+    // the LDS writes
+    for (uint32_t i00 = 0; i00 < n; i00 += kStageU * kWG) {
+      int32_t cf[kStageU], cg[kStageU];
+      uint32_t sg[kStageU], rc[kStageU], gd[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        const uint32_t ia = i0 + (i < n ? i : 0u);
+        cf[u] = B.it_cf[ia];
+        cg[u] = B.it_cg[ia];
+        sg[u] = B.it_seg[ia];
+        rc[u] = B.it_rec[ia];
+        gd[u] = Scaled ? B.it_gid[ia] : 0u;
+      }
+      Real vf[kStageU], vg[kStageU];
+      int ev[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        vf[u] = ell[cf[u] >= 0 ? cf[u] : 0];
+        vg[u] = ell[cg[u] >= 0 ? cg[u] : 0];
+        ev[u] = Scaled ? E[gd[u]] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        if (i < n) {
+          sSegL[i] = (uint16_t)(sg[u] - S.seg0);
+          bF[i] = cf[u] >= 0 ? vf[u] : Real(0);
+          bB[i] = cg[u] >= 0 ? vg[u] : Real(0);  // child1 tops (bB is free until row 1)
+          sAcc[i] = 0;
+          sRO[i] = rc[u] - rb;
+          if (Scaled) {
+            sEi[i] = ev[u];
+            sFlag[i] = ev[u] == kUndef ? (kPending | kYoung) : 0;
+          }
+        }
       }
     }
     if (lane == 0) sRO[n] = nrec;
-#pragma unroll 4
-    for (uint32_t k = lane; k < nrec; k += kWG) {
-      sOps[k] = B.rec_ops[rb + k];
-      sOwn[k] = B.rec_own[rb + k];
-      const Real p = (Real)B.rec_p[rb + k];
-      sCoef[k] = p;  // the scaled forms rescale it after row 0
-      sP[k] = p;
+    for (uint32_t k00 = 0; k00 < nrec; k00 += kStageU * kWG) {
+      uint32_t op[kStageU];
+      uint16_t ow[kStageU];
+      double pp[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        const uint32_t ka = rb + (k < nrec ? k : 0u);
+        op[u] = B.rec_ops[ka];
+        ow[u] = B.rec_own[ka];
+        pp[u] = B.rec_p[ka];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        if (k < nrec) {
+          sOps[k] = op[u];
+          sOwn[k] = ow[u];
+          const Real p = (Real)pp[u];
+          sCoef[k] = p;  // the scaled forms rescale it after row 0
+          sP[k] = p;
+        }
+      }
     }
     wsync<Gws>();
     // Row 0 of a stage without WGD segments: the split terms are summed
@@ -986,7 +1045,7 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
           v = ldexp_r(v, -sh);
           if (StoreAll)
             for (uint32_t q = 0; q < r; ++q) {
-              Real* p = ell + it.cell + (size_t)q * it.stride;
+              const gptr<Real> p = ell + it.cell + (size_t)q * it.stride;
               *p = ldexp_r(*p, -sh);
             }
           bF[i] = (Real)sh;  // bF is free after row 0
@@ -1123,20 +1182,60 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
     uint16_t* const sOwn = T.sOwn;
     uint32_t* const sRO = T.sRO;
     int32_t* const sEi = T.sEi;
-    for (uint32_t i = lane; i < n; i += kWG) {
-      const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
-      bF[i] = cf >= 0 ? ell[cf] : Real(0);
-      bB[i] = cg >= 0 ? ell[cg] : Real(0);
-      sRO[i] = B.it_rec[i0 + i] - rb;
-      if (Scaled) sEi[i] = E[i];  // root items are the clades in order
+    for (uint32_t i00 = 0; i00 < n; i00 += kStageU * kWG) {
+      int32_t cf[kStageU], cg[kStageU];
+      uint32_t rc[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        const uint32_t ia = i0 + (i < n ? i : 0u);
+        cf[u] = B.it_cf[ia];
+        cg[u] = B.it_cg[ia];
+        rc[u] = B.it_rec[ia];
+      }
+      Real vf[kStageU], vg[kStageU];
+      int ev[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        vf[u] = ell[cf[u] >= 0 ? cf[u] : 0];
+        vg[u] = ell[cg[u] >= 0 ? cg[u] : 0];
+        ev[u] = Scaled ? E[i < n ? i : 0u] : 0;  // root items are the clades in order
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        if (i < n) {
+          bF[i] = cf[u] >= 0 ? vf[u] : Real(0);
+          bB[i] = cg[u] >= 0 ? vg[u] : Real(0);
+          sRO[i] = rc[u] - rb;
+          if (Scaled) sEi[i] = ev[u];
+        }
+      }
     }
     if (lane == 0) sRO[n] = nrec;
     if (inl) {
-#pragma unroll 4
-      for (uint32_t k = lane; k < nrec; k += kWG) {
-        sOps[k] = B.rec_ops[rb + k];
-        sOwn[k] = B.rec_own[rb + k];
-        sCoef[k] = (Real)B.rec_p[rb + k];
+      for (uint32_t k00 = 0; k00 < nrec; k00 += kStageU * kWG) {
+        uint32_t op[kStageU];
+        uint16_t ow[kStageU];
+        double pp[kStageU];
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+          const uint32_t k = k00 + u * kWG + lane;
+          const uint32_t ka = rb + (k < nrec ? k : 0u);
+          op[u] = B.rec_ops[ka];
+          ow[u] = B.rec_own[ka];
+          pp[u] = B.rec_p[ka];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+          const uint32_t k = k00 + u * kWG + lane;
+          if (k < nrec) {
+            sOps[k] = op[u];
+            sOwn[k] = ow[u];
+            sCoef[k] = (Real)pp[u];
+          }
+        }
       }
     }
     const LevelBounds LB(B, F, lane);
@@ -1385,8 +1484,11 @@ template <typename Real, bool Scaled, bool TabL, bool Gws>
 __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M, const WorkDev<Real>& W,
                                            const LdsCfg& L, uint32_t f, unsigned char* const wb, Real* sTab,
                                            Real* const accPS, double* const accX, const NodeC<Real>& NC,
-                                           uint32_t lane, const Real* ell, Real* ibar, const Real* coefs,
-                                           uint32_t coef_base) {
+                                           uint32_t lane, const Real* ell_arg, Real* ibar_arg,
+                                           const Real* coefs_arg, uint32_t coef_base) {
+  const gptr<const Real> ell = glob(ell_arg);
+  const gptr<Real> ibar = glob(ibar_arg);
+  const gptr<const Real> coefs = glob(coefs_arg);
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
   const AccLayout AL(M.tab_rows, M.n, M.n_wgd);
   // Deterministic gradients: this family's accumulators go to its own slot
@@ -1424,20 +1526,60 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
     uint16_t* const sOwn = T.sOwn;
     uint32_t* const sRO = T.sRO;
     const LevelBounds LB(B, F, lane);
-    for (uint32_t i = lane; i < n; i += kWG) {
-      const int32_t cf = B.it_cf[i0 + i], cg = B.it_cg[i0 + i];
-      bV[i] = ell[sg.ell + i];
-      bB[i] = cf >= 0 ? ell[cf] : Real(0);  // child0 tops
-      bG[i] = cg >= 0 ? ell[cg] : Real(0);  // child1 tops
-      bA[i] = 0;
-      sRO[i] = B.it_rec[i0 + i] - rb;
+    for (uint32_t i00 = 0; i00 < n; i00 += kStageU * kWG) {
+      int32_t cf[kStageU], cg[kStageU];
+      uint32_t rc[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        const uint32_t ia = i0 + (i < n ? i : 0u);
+        cf[u] = B.it_cf[ia];
+        cg[u] = B.it_cg[ia];
+        rc[u] = B.it_rec[ia];
+      }
+      Real vv[kStageU], vf[kStageU], vg[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        vv[u] = ell[sg.ell + (i < n ? i : 0u)];
+        vf[u] = ell[cf[u] >= 0 ? cf[u] : 0];
+        vg[u] = ell[cg[u] >= 0 ? cg[u] : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        if (i < n) {
+          bV[i] = vv[u];
+          bB[i] = cf[u] >= 0 ? vf[u] : Real(0);  // child0 tops
+          bG[i] = cg[u] >= 0 ? vg[u] : Real(0);  // child1 tops
+          bA[i] = 0;
+          sRO[i] = rc[u] - rb;
+        }
+      }
     }
     if (lane == 0) sRO[n] = nrec;
     if (inl)
-      for (uint32_t k = lane; k < nrec; k += kWG) {
-        sOps[k] = B.rec_ops[rb + k];
-        sOwn[k] = B.rec_own[rb + k];
-        sCoef[k] = coef_at(rb + k);
+      for (uint32_t k00 = 0; k00 < nrec; k00 += kStageU * kWG) {
+        uint32_t op[kStageU];
+        uint16_t ow[kStageU];
+        Real cc[kStageU];
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+          const uint32_t k = k00 + u * kWG + lane;
+          const uint32_t ka = rb + (k < nrec ? k : 0u);
+          op[u] = B.rec_ops[ka];
+          ow[u] = B.rec_own[ka];
+          cc[u] = Scaled ? coefs[ka - coef_base] : (Real)B.rec_p[ka];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageU; ++u) {
+          const uint32_t k = k00 + u * kWG + lane;
+          if (k < nrec) {
+            sOps[k] = op[u];
+            sOwn[k] = ow[u];
+            sCoef[k] = cc[u];
+          }
+        }
       }
     wsync<Gws>();
     // record k: operands, coefficient, owning clade
@@ -1537,18 +1679,55 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
     wave_sync_mem();  // ī written by the stage above
     SegTab ST;
     ST.load(B, M, S.seg0, S.n_segs, lane);
-    for (uint32_t i = lane; i < n; i += kWG) {
-      bA[i] = ibar[B.it_bar[i0 + i]];
-      bV[i] = 0;
-      sRO[i] = B.it_rec[i0 + i] - rb;
-      sSegL[i] = (uint16_t)(B.it_seg[i0 + i] - S.seg0);
+    // kStageU items (records) per lane at once: their index loads, then the
+    // loads of ī they name, then the LDS writes
+    for (uint32_t i00 = 0; i00 < n; i00 += kStageU * kWG) {
+      int32_t ib[kStageU];
+      uint32_t rc[kStageU], sg[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        const uint32_t ia = i0 + (i < n ? i : 0u);
+        ib[u] = B.it_bar[ia];
+        rc[u] = B.it_rec[ia];
+        sg[u] = B.it_seg[ia];
+      }
+      Real vb[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) vb[u] = ibar[ib[u]];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t i = i00 + u * kWG + lane;
+        if (i < n) {
+          bA[i] = vb[u];
+          bV[i] = 0;
+          sRO[i] = rc[u] - rb;
+          sSegL[i] = (uint16_t)(sg[u] - S.seg0);
+        }
+      }
     }
     if (lane == 0) sRO[n] = nrec;
-#pragma unroll 4
-    for (uint32_t k = lane; k < nrec; k += kWG) {
-      sOps[k] = B.rec_ops[rb + k];
-      sOwn[k] = B.rec_own[rb + k];
-      sCoef[k] = coef_at(rb + k);
+    for (uint32_t k00 = 0; k00 < nrec; k00 += kStageU * kWG) {
+      uint32_t op[kStageU];
+      uint16_t ow[kStageU];
+      Real cc[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        const uint32_t ka = rb + (k < nrec ? k : 0u);
+        op[u] = B.rec_ops[ka];
+        ow[u] = B.rec_own[ka];
+        cc[u] = Scaled ? coefs[ka - coef_base] : (Real)B.rec_p[ka];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        if (k < nrec) {
+          sOps[k] = op[u];
+          sOwn[k] = ow[u];
+          sCoef[k] = cc[u];
+        }
+      }
     }
     wsync<Gws>();
     ItemRow cache[kCacheB];
