@@ -642,10 +642,11 @@ struct FwdRoot {
 // forward
 // ---------------------------------------------------------------------------
 constexpr int kMaxFPW = 8;  // families (waves) per workgroup, at most
-// resident waves per SIMD the DP kernels are compiled for (registers ≤ 512/3):
-// a wave mostly waits on its own LDS round trips, so a third wave per SIMD
-// hides more of them than the registers it costs
-constexpr int kWavesPerEU = 3;
+// resident waves per SIMD the DP kernels are compiled for (registers ≤ 512/2):
+// the stage buffers in LDS hold about two waves per SIMD of most size
+// classes anyway, and at three (≤ 168 registers) the reverse sweep and the
+// FP64 forms spill to scratch
+constexpr int kWavesPerEU = 2;
 
 // a mixture launch's workgroup takes its component's model, workspace and,
 // when given, family order and count
