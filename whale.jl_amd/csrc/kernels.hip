@@ -355,10 +355,13 @@ __device__ __forceinline__ RunMask run_mask(uint32_t key) {
 }
 template <typename Real>
 __device__ __forceinline__ Real run_sum(Real v, const RunMask& m) {
-  if (m.steps > 0) v = fma(dpp_shr<1>(v), (Real)m.s1, v);
-  if (m.steps > 1) v = fma(dpp_shr<2>(v), (Real)m.s2, v);
-  if (m.steps > 2) v = fma(dpp_shr<4>(v), (Real)m.s4, v);
-  if (m.steps > 3) v = fma(dpp_shr<8>(v), (Real)m.s8, v);
+  // the count is uniform: read it into a scalar register so that the tests
+  // are scalar branches, not exec-mask changes around each shift
+  const int st = __builtin_amdgcn_readfirstlane(m.steps);
+  if (st > 0) v = fma(dpp_shr<1>(v), (Real)m.s1, v);
+  if (st > 1) v = fma(dpp_shr<2>(v), (Real)m.s2, v);
+  if (st > 2) v = fma(dpp_shr<4>(v), (Real)m.s4, v);
+  if (st > 3) v = fma(dpp_shr<8>(v), (Real)m.s8, v);
   return v;
 }
 
