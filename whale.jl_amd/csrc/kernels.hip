@@ -479,6 +479,23 @@ __device__ __forceinline__ gptr<T> glob(T* p) {
 // items (records) per lane that a stage's set-up loads at once
 constexpr int kStageU = 4;
 
+// A record every lane of the wave loaded from the same address, moved to
+// scalar registers.  The kernels write global memory, so the compiler loads
+// such records with vector loads and treats their fields as varying across
+// lanes; tests on them (item and record counts, row counts) would then be
+// exec-mask branches instead of scalar ones.
+template <typename T>
+__device__ __forceinline__ T uniform(const T& x) {
+  static_assert(sizeof(T) % 4 == 0, "a record of whole dwords");
+  uint32_t w[sizeof(T) / 4];
+  __builtin_memcpy(w, &x, sizeof(T));
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) w[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)w[k]);
+  T y;
+  __builtin_memcpy(&y, w, sizeof(T));
+  return y;
+}
+
 struct SegTab {
   SegInfo mine;
   uint32_t seg0;
@@ -674,13 +691,13 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
   const gptr<Real> ell = glob(ell_arg);
   const gptr<Real> coef = glob(coef_arg);
   constexpr int RC = 2;  // records per lane held in registers (128 per stage)
-  const FamDev F = B.fam[f];
+  const FamDev F = uniform(B.fam[f]);
   const gptr<int32_t> E = glob(W.E + F.clade0);
   if (Scaled)
     for (uint32_t c = lane; c < F.n_clades; c += kWG) E[c] = kUndef;
   // ---------------- non-root stages
   for (uint32_t s = 0; s < F.n_stages; ++s) {
-    const StageDev S = B.stage[F.stage0 + s];
+    const StageDev S = uniform(B.stage[F.stage0 + s]);
     const uint32_t n = S.n_items, i0 = S.item0;
     const uint32_t rb = S.rec0, nrec = S.n_recs;
     const FwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
@@ -1166,7 +1183,7 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
     wave_sync_mem();
     const uint32_t n = F.n_clades;
     const uint32_t i0 = F.root_item0;
-    const SegDev sg = B.seg[F.root_seg];
+    const SegDev sg = uniform(B.seg[F.root_seg]);
     const NodeDev& nd = M.node[M.root];
     const double eta = M.par[3 * M.n + M.n_wgd];
     const double er = eps_top(M, M.root);
@@ -1504,13 +1521,13 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
     for (int j = (int)lane; j < AL.n; j += kWG) accF[j] = 0.0;
     wave_sync_mem();
   }
-  const FamDev F = B.fam[f];
+  const FamDev F = uniform(B.fam[f]);
   auto coef_at = [&](uint32_t k) -> Real { return Scaled ? coefs[k - coef_base] : (Real)B.rec_p[k]; };
   // ---------------- root
   {
     const uint32_t n = F.n_clades;
     const uint32_t i0 = F.root_item0;
-    const SegDev sg = B.seg[F.root_seg];
+    const SegDev sg = uniform(B.seg[F.root_seg]);
     const NodeDev& nd = M.node[M.root];
     const double eta = M.par[3 * M.n + M.n_wgd];
     const double er = eps_top(M, M.root);
@@ -1666,7 +1683,7 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
   }
   // ---------------- non-root stages, top down
   for (int s = (int)F.n_stages - 1; s >= 0; --s) {
-    const StageDev S = B.stage[F.stage0 + s];
+    const StageDev S = uniform(B.stage[F.stage0 + s]);
     const uint32_t n = S.n_items, i0 = S.item0;
     const uint32_t rb = S.rec0, nrec = S.n_recs;
     const BwdStage<Real> T(wb, n, nrec);  // fits the wave's LDS (size class)
