@@ -716,12 +716,42 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
     wave_sync_mem();  // ℓ and exponents of the stages below
     SegTab ST;
     ST.load(B, M, S.seg0, S.n_segs, lane);
-    // kStageU items per lane at once: all their index loads, then all the
-    // loads of ℓ and exponents they name (in-bounds dummies past n), then
-    // the LDS writes
-    for (uint32_t i00 = 0; i00 < n; i00 += kStageU * kWG) {
-      int32_t cf[kStageU], cg[kStageU];
-      uint32_t sg[kStageU], rc[kStageU], gd[kStageU];
+    // kStageU items (records) per lane at once: all their index loads, then
+    // all the loads of ℓ and exponents they name (in-bounds dummies past n),
+    // then the LDS writes.  The first batches of records and items go
+    // together, so a stage of up to 256 items and 256 records is set up in
+    // two dependent round trips to memory.
+    uint32_t op[kStageU];
+    uint16_t ow[kStageU];
+    double pp[kStageU];
+    auto rec_load = [&](uint32_t k00) {
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        const uint32_t ka = rb + (k < nrec ? k : 0u);
+        op[u] = B.rec_ops[ka];
+        ow[u] = B.rec_own[ka];
+        pp[u] = B.rec_p[ka];
+      }
+    };
+    auto rec_put = [&](uint32_t k00) {
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        if (k < nrec) {
+          sOps[k] = op[u];
+          sOwn[k] = ow[u];
+          const Real p = (Real)pp[u];
+          sCoef[k] = p;  // the scaled forms rescale it after row 0
+          sP[k] = p;
+        }
+      }
+    };
+    int32_t cf[kStageU], cg[kStageU];
+    uint32_t sg[kStageU], rc[kStageU], gd[kStageU];
+    Real vf[kStageU], vg[kStageU];
+    int ev[kStageU];
+    auto item_load = [&](uint32_t i00) {
 #pragma unroll
       for (int u = 0; u < kStageU; ++u) {
         const uint32_t i = i00 + u * kWG + lane;
@@ -732,14 +762,16 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
         rc[u] = B.it_rec[ia];
         gd[u] = Scaled ? B.it_gid[ia] : 0u;
       }
-      Real vf[kStageU], vg[kStageU];
-      int ev[kStageU];
+    };
+    auto item_gather = [&]() {
 #pragma unroll
       for (int u = 0; u < kStageU; ++u) {
         vf[u] = ell[cf[u] >= 0 ? cf[u] : 0];
         vg[u] = ell[cg[u] >= 0 ? cg[u] : 0];
         ev[u] = Scaled ? E[gd[u]] : 0;
       }
+    };
+    auto item_put = [&](uint32_t i00) {
 #pragma unroll
       for (int u = 0; u < kStageU; ++u) {
         const uint32_t i = i00 + u * kWG + lane;
@@ -755,32 +787,22 @@ __device__ __forceinline__ void fwd_family(const BatchDev& B, const ModelDev& M,
           }
         }
       }
+    };
+    if (nrec > 0) rec_load(0);
+    item_load(0);
+    item_gather();
+    if (nrec > 0) rec_put(0);
+    item_put(0);
+    for (uint32_t i00 = kStageU * kWG; i00 < n; i00 += kStageU * kWG) {
+      item_load(i00);
+      item_gather();
+      item_put(i00);
+    }
+    for (uint32_t k00 = kStageU * kWG; k00 < nrec; k00 += kStageU * kWG) {
+      rec_load(k00);
+      rec_put(k00);
     }
     if (lane == 0) sRO[n] = nrec;
-    for (uint32_t k00 = 0; k00 < nrec; k00 += kStageU * kWG) {
-      uint32_t op[kStageU];
-      uint16_t ow[kStageU];
-      double pp[kStageU];
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const uint32_t k = k00 + u * kWG + lane;
-        const uint32_t ka = rb + (k < nrec ? k : 0u);
-        op[u] = B.rec_ops[ka];
-        ow[u] = B.rec_own[ka];
-        pp[u] = B.rec_p[ka];
-      }
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const uint32_t k = k00 + u * kWG + lane;
-        if (k < nrec) {
-          sOps[k] = op[u];
-          sOwn[k] = ow[u];
-          const Real p = (Real)pp[u];
-          sCoef[k] = p;  // the scaled forms rescale it after row 0
-          sP[k] = p;
-        }
-      }
-    }
     wsync<Gws>();
     // Row 0 of a stage without WGD segments: the split terms are summed
     // record-parallel first (into sAcc, which is zero until the row loop),
@@ -1701,10 +1723,36 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
     SegTab ST;
     ST.load(B, M, S.seg0, S.n_segs, lane);
     // kStageU items (records) per lane at once: their index loads, then the
-    // loads of ī they name, then the LDS writes
-    for (uint32_t i00 = 0; i00 < n; i00 += kStageU * kWG) {
-      int32_t ib[kStageU];
-      uint32_t rc[kStageU], sg[kStageU];
+    // loads of ī they name, then the LDS writes; the first batches of
+    // records and items go together (two dependent round trips to memory)
+    uint32_t op[kStageU];
+    uint16_t ow[kStageU];
+    Real cc[kStageU];
+    auto rec_load = [&](uint32_t k00) {
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        const uint32_t ka = rb + (k < nrec ? k : 0u);
+        op[u] = B.rec_ops[ka];
+        ow[u] = B.rec_own[ka];
+        cc[u] = Scaled ? coefs[ka - coef_base] : (Real)B.rec_p[ka];
+      }
+    };
+    auto rec_put = [&](uint32_t k00) {
+#pragma unroll
+      for (int u = 0; u < kStageU; ++u) {
+        const uint32_t k = k00 + u * kWG + lane;
+        if (k < nrec) {
+          sOps[k] = op[u];
+          sOwn[k] = ow[u];
+          sCoef[k] = cc[u];
+        }
+      }
+    };
+    int32_t ib[kStageU];
+    uint32_t rc[kStageU], sg[kStageU];
+    Real vb[kStageU];
+    auto item_load = [&](uint32_t i00) {
 #pragma unroll
       for (int u = 0; u < kStageU; ++u) {
         const uint32_t i = i00 + u * kWG + lane;
@@ -1713,9 +1761,12 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
         rc[u] = B.it_rec[ia];
         sg[u] = B.it_seg[ia];
       }
-      Real vb[kStageU];
+    };
+    auto item_gather = [&]() {
 #pragma unroll
       for (int u = 0; u < kStageU; ++u) vb[u] = ibar[ib[u]];
+    };
+    auto item_put = [&](uint32_t i00) {
 #pragma unroll
       for (int u = 0; u < kStageU; ++u) {
         const uint32_t i = i00 + u * kWG + lane;
@@ -1726,30 +1777,22 @@ __device__ __forceinline__ void bwd_family(const BatchDev& B, const ModelDev& M,
           sSegL[i] = (uint16_t)(sg[u] - S.seg0);
         }
       }
+    };
+    if (nrec > 0) rec_load(0);
+    item_load(0);
+    item_gather();
+    if (nrec > 0) rec_put(0);
+    item_put(0);
+    for (uint32_t i00 = kStageU * kWG; i00 < n; i00 += kStageU * kWG) {
+      item_load(i00);
+      item_gather();
+      item_put(i00);
+    }
+    for (uint32_t k00 = kStageU * kWG; k00 < nrec; k00 += kStageU * kWG) {
+      rec_load(k00);
+      rec_put(k00);
     }
     if (lane == 0) sRO[n] = nrec;
-    for (uint32_t k00 = 0; k00 < nrec; k00 += kStageU * kWG) {
-      uint32_t op[kStageU];
-      uint16_t ow[kStageU];
-      Real cc[kStageU];
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const uint32_t k = k00 + u * kWG + lane;
-        const uint32_t ka = rb + (k < nrec ? k : 0u);
-        op[u] = B.rec_ops[ka];
-        ow[u] = B.rec_own[ka];
-        cc[u] = Scaled ? coefs[ka - coef_base] : (Real)B.rec_p[ka];
-      }
-#pragma unroll
-      for (int u = 0; u < kStageU; ++u) {
-        const uint32_t k = k00 + u * kWG + lane;
-        if (k < nrec) {
-          sOps[k] = op[u];
-          sOwn[k] = ow[u];
-          sCoef[k] = cc[u];
-        }
-      }
-    }
     wsync<Gws>();
     ItemRow cache[kCacheB];
     ItemTop tops[kCacheB];
