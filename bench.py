@@ -211,6 +211,7 @@ def main():
     ap.add_argument("--tune-cache", default=os.path.join(os.path.expanduser("~"), ".cache", "whale_amd", "tuning.txt"),
                     help="autotuner cache file (empty: search every time)")
     ap.add_argument("--sampler-points", type=int, default=256)
+    ap.add_argument("--tune-reps", type=int, default=5, help="timed replays per candidate shape in the search")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device-to-device copy of 256 MiB (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -273,7 +274,7 @@ def main():
         cache = args.tune_cache or None
         if cache:
             os.makedirs(os.path.dirname(cache), exist_ok=True)
-        tune = batch.autotune("fp32", gradient=True, reps=5, cache=cache)
+        tune = batch.autotune("fp32", gradient=True, reps=args.tune_reps, cache=cache)
         tune["tuning_of"] = "class LDS bounds (bytes), stream layout, waves per CU; ms are host-timed graph replays"
         log(f"autotune: {tune}")
     for r in batch.launches("fp32"):  # the shape the timed steps run

@@ -1736,31 +1736,60 @@ int whale_batch_autotune(whale_batch* b, int32_t mode, int32_t gradient, int32_t
       }
       return WHALE_OK;
     };
-    // two rounds: the best layout depends on the class bounds and the other
-    // way round, so the layouts are tried again once the bounds have moved
-    for (int round = 0; round < 2; ++round) {
-      const Tuning before = best;
-      for (int l = 0; l <= (kPoolEnabled ? kLayoutPool : kLayouts - 1); ++l) {
-        if (l == best.layout) continue;
-        Tuning t = best;
-        t.layout = l;
-        if ((rc = consider(t)) != WHALE_OK) return rc;
-      }
-      for (int cap : {4, 6, 8, 12, 16}) {
-        if (cap == best.waves_cap) continue;
-        Tuning t = best;
-        t.waves_cap = cap;
-        if ((rc = consider(t)) != WHALE_OK) return rc;
-      }
-      for (int pass = 0; pass < 2; ++pass)
-        for (int i = 0; i < 5; ++i)
-          for (double f : {0.75, 0.9, 1.12, 1.33}) {
+    // the layouts at the starting bounds, then a descent from each of the two
+    // fastest: their basins differ, and at the starting bounds the two are
+    // often within the timing noise of each other
+    std::vector<std::pair<double, int>> lay{{t_start, start.layout}};
+    for (int l = 0; l <= (kPoolEnabled ? kLayoutPool : kLayouts - 1); ++l) {
+      if (l == start.layout) continue;
+      Tuning t = start;
+      t.layout = l;
+      if (!valid_tuning(t)) continue;
+      double ms = 0;
+      if ((rc = timed(t, &ms)) != WHALE_OK) return rc;
+      lay.push_back({ms, l});
+    }
+    std::sort(lay.begin(), lay.end());
+    Tuning overall = start;
+    double overall_ms = t_start;
+    for (size_t d = 0; d < std::min<size_t>(2, lay.size()); ++d) {
+      best = start;
+      best.layout = lay[d].second;
+      t_best = lay[d].first;
+      // two rounds: the best layout depends on the class bounds and the
+      // other way round, so the layouts are tried again once the bounds
+      // have moved
+      for (int round = 0; round < 2; ++round) {
+        const Tuning before = best;
+        if (round > 0)
+          for (int l = 0; l <= (kPoolEnabled ? kLayoutPool : kLayouts - 1); ++l) {
+            if (l == best.layout) continue;
             Tuning t = best;
-            t.bounds[i] = (uint32_t)((double)best.bounds[i] * f) & ~255u;
+            t.layout = l;
             if ((rc = consider(t)) != WHALE_OK) return rc;
           }
-      if (round == 0 && std::memcmp(&before, &best, sizeof(Tuning)) == 0) break;  // nothing moved
+        for (int cap : {4, 6, 8, 12, 16}) {
+          if (cap == best.waves_cap) continue;
+          Tuning t = best;
+          t.waves_cap = cap;
+          if ((rc = consider(t)) != WHALE_OK) return rc;
+        }
+        for (int pass = 0; pass < 2; ++pass)
+          for (int i = 0; i < 5; ++i)
+            for (double f : {0.75, 0.9, 1.12, 1.33}) {
+              Tuning t = best;
+              t.bounds[i] = (uint32_t)((double)best.bounds[i] * f) & ~255u;
+              if ((rc = consider(t)) != WHALE_OK) return rc;
+            }
+        if (round == 0 && std::memcmp(&before, &best, sizeof(Tuning)) == 0) break;  // nothing moved
+      }
+      if (t_best < overall_ms) {
+        overall = best;
+        overall_ms = t_best;
+      }
     }
+    best = overall;
+    t_best = overall_ms;
     b->tune = best;
     if ((rc = replan(b)) != WHALE_OK) return rc;
     cache_store(cache_path, key, best, t_best);
