@@ -14,6 +14,7 @@ throughput on a bounded sample.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config4]
                     [--no-cpu] [--no-per-family] [--cpu-seconds S]
+                    [--no-sampler] [--no-tune] [--tune-reps R] [--tune-cache FILE]
 """
 import argparse
 import ctypes as C
